@@ -1,0 +1,156 @@
+/*
+ * mpc.h -- device-resident API of libmpc_hip.so.
+ *
+ * The reference has no counterpart for these entry points: its propagator
+ * copies the whole field host<->device on every call (propagation.c:6-8,59,
+ * 180-210).  Here the field stays in HBM for a whole run; the host sees it
+ * only at upload, dumps and checkpoints.  The legacy entry points in
+ * propagation.h / plasma.h are thin wrappers over this API.
+ *
+ * All functions return 0 on success and a negative MPC_E* code on error;
+ * mpc_strerror() gives a message.  Nothing here falls back to the CPU.
+ */
+#ifndef MPC_MPC_H
+#define MPC_MPC_H
+#include "settings.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPC_OK          0
+#define MPC_EINVAL    (-1)
+#define MPC_ENOMEM    (-2)
+#define MPC_EHIP      (-3)
+#define MPC_ENODEV    (-4)
+#define MPC_EUNSUPP   (-5)
+#define MPC_EIO       (-6)
+
+#define MPC_F64 0   /* plain FP64 path */
+#define MPC_F32 1   /* FP32 complex with FP64 islands */
+
+/*
+ * Run parameters: the content of param.bin (reference MPC.c:180-203, same
+ * meaning and units) plus runtime options param.bin does not hold.
+ * Arrays are host pointers, read at mpc_create().
+ */
+typedef struct {
+    int    dims[3];          /* Nx, Ny, Nt; Nx == Ny (reference propagation.c:83,95,386) */
+    int    sst_on, plasm;
+    int    steps_per_pass;   /* fixed-step schedule: z-steps per cell pass */
+    int    npasses;
+    double dist, dzz, alpha, Cav, k0, gamma2, w0, deltat;
+    double rho_c, rho_nt, n0, puls_e, Ab_M, dr, def_err, dzmin, Esig;
+    double mirror_R;         /* 0: reference sign-flip mirror; >0: curved mirror radius (m) */
+    double aperture;         /* 0: none; >0: hard circular aperture radius (m) */
+    const double *betas;     /* [Nt] dispersion on the ascending centred Omega grid */
+    const double *wr;        /* [Ny] ascending centred kx grid */
+    const double *w;         /* [Nt] ascending centred Omega grid */
+    const double *r;         /* [Ny] transverse x grid */
+    char   gas[16];          /* "Argon", "Helium", "Neon", "Air", "Nitrogen", "Xenon", "Krypton" */
+    unsigned quirks;         /* must be 0 on the device (the corrected arithmetic) */
+    int    pad_;
+} mpc_params;
+
+/* Diagnostics of the resident field (reference MPC.c:384-512 outputs plus
+ * per-pass observables).  Arrays are host pointers the caller sizes; any may
+ * be NULL. */
+typedef struct {
+    double energy_sum;       /* sum |A|^2 */
+    double rho_peak;         /* rho_nt * max(ions1 + 2 ions2) of the last nonlinear step */
+    double spec_rms_width;   /* rms width of spec over the FFT-order Omega grid (rad/s) */
+    double *temp;            /* [Nt] sum_xy |A|^2 */
+    double *spec;            /* [Nt] sum_xy |FFT_t A|^2, FFT order, unnormalised */
+    double *BP1;             /* [Nx] sum_{y,t} |A|^2 */
+    double *BP2;             /* [Ny] sum_{x,t} |A|^2 */
+} mpc_diag;
+
+typedef struct mpc_ctx_s mpc_ctx;
+
+const char *mpc_strerror(int code);
+int  mpc_last_error(void);               /* sticky error of the legacy void entry points */
+int  mpc_sizeof_params(void);
+int  mpc_device_count(void);
+
+int  mpc_create(const mpc_params *p, int precision, mpc_ctx **out);
+void mpc_destroy(mpc_ctx *ctx);
+/* field transfer, reference layout A[t][y][x] (x fastest) */
+int  mpc_upload(mpc_ctx *ctx, const cufftDoubleComplex *A);
+int  mpc_download(mpc_ctx *ctx, cufftDoubleComplex *A);
+/* scale the resident field by sqrt(puls_e/Esig) (reference MPC.c:206-216,
+ * both parts: the reference scales Re only) */
+int  mpc_calibrate(mpc_ctx *ctx);
+/*
+ * Fixed-step run of nsteps z-steps starting at global step index step0:
+ * each step is L(dzz/2) N(dzz) L(dzz/2) evaluated at z = (n + 1/2) dzz,
+ * with the mirror applied between passes.  Consecutive half steps are merged
+ * (3 HBM sweeps per interior step).  On return the field is in the time
+ * domain again.
+ */
+int  mpc_run(mpc_ctx *ctx, int step0, int nsteps);
+/* One isolated split step at z with step dzz (reference ssfmprop semantics,
+ * signmem updated as calcexpR does). */
+int  mpc_step(mpc_ctx *ctx, double dzz, double z, int *signmem);
+/* mirror (curvature x aperture x sqrt(Ab_M)) applied once to the field */
+int  mpc_mirror(mpc_ctx *ctx);
+int  mpc_diagnostics(mpc_ctx *ctx, mpc_diag *d);
+/* adaptive step doubling + Richardson (reference MPC.c:343-383,
+ * propagation.c:616-677) for at most max_steps accepted steps.  z, dzz and
+ * signmem are updated; errs/dzs (may be NULL) receive per accepted step. */
+int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int max_steps,
+                      int *accepted, double *errs, double *dzs);
+
+/* Timing of the most recent mpc_run: average device time per launch of each
+ * sweep kernel, measured with HIP events on the library's stream.
+ * which: 0 = t-line, 1 = x-line, 2 = y-line.  Returns ms, or <0. */
+int  mpc_set_timing(mpc_ctx *ctx, int enable);
+double mpc_kernel_ms(mpc_ctx *ctx, int which);
+long long mpc_field_bytes(mpc_ctx *ctx);
+
+/*
+ * Input generator (synthetic workloads): fills the param.bin content and the
+ * unscaled input field for a Gaussian pulse in x, y, t with seeded complex
+ * Gaussian noise (SplitMix64 + Box-Muller).  Arrays are caller-allocated:
+ * betas[Nt], wr[Ny], w[Nt], r[Ny], Are[N], Aim[N].
+ */
+typedef struct {
+    int    nx, ny, nt;
+    int    passes, steps_per_pass;
+    int    sst_on, plasm;
+    int    pad_;
+    double window_xy;        /* m, full transverse window */
+    double window_t;         /* s, full time window */
+    double lambda0;          /* m */
+    double waist;            /* m, 1/e field radius exp(-(x^2+y^2)/waist^2) */
+    double tau_fwhm;         /* s, intensity FWHM */
+    double energy;           /* J */
+    double noise_rel;        /* noise sigma per component / max|A| */
+    double n2_per_bar;       /* m^2/W at 1 bar */
+    double beta2;            /* s^2/m */
+    double pressure;         /* bar */
+    double rho_nt_per_bar;   /* m^-3 at 1 bar */
+    double cell_L;           /* m */
+    double mirror_R, aperture, Ab_M, def_err, dzmin;
+    unsigned long long seed;
+    char   gas[16];
+} mpc_input_spec;
+
+/* ADK constants Kon1..Kon6 of a gas (plasma.c:45-51 with the quantum
+ * numbers of :159-174, corrected arithmetic); 0 if the gas is known */
+int mpc_adk_constants(const char *gas, double *kon);
+
+int mpc_mkinput(const mpc_input_spec *s, mpc_params *p, double *betas, double *wr,
+                double *w, double *r, double *Are, double *Aim);
+int mpc_sizeof_input_spec(void);
+
+/* param.bin / field file I/O (reference MPC.c:76-77,159-168,180-203 formats) */
+int mpc_write_param_bin(const char *path, const mpc_params *p);
+/* reads param.bin; arrays are malloc'd here and returned through p (free
+ * with mpc_free_params) */
+int mpc_read_param_bin(const char *path, mpc_params *p);
+void mpc_free_params(mpc_params *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

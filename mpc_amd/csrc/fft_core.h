@@ -1,0 +1,216 @@
+/*
+ * fft_core.h -- register-resident Stockham FFT building blocks for CDNA4.
+ *
+ * This replaces the reference's cuFFT/FFTW plans (propagation.c:508-613) for
+ * the three line geometries (x, y, t).  Conventions are FFTW's: forward
+ * (DIR = -1) is exp(-2 pi i jk/n), inverse (DIR = +1) exp(+...), both
+ * unnormalised.
+ *
+ * Work split: a line of N points is owned by TPL = N/E threads; thread tid
+ * holds x[tid + TPL*m] for m in [0, E) in registers.  A stage of radix R
+ * does E/R butterflies per thread in registers, then writes its outputs to
+ * the line's LDS buffer in Stockham order; after a barrier every thread
+ * reads back x[tid + TPL*m].  The last stage's outputs land in the same
+ * register slots they were read from, so a transform of S stages costs
+ * S-1 LDS round trips, and both the first load and the last store use the
+ * coalesced pattern x[tid + TPL*m].
+ *
+ * Everything here is host+device so the index arithmetic can be exercised
+ * on the CPU by a loop-over-threads emulation (tests/).
+ */
+#ifndef MPC_FFT_CORE_H
+#define MPC_FFT_CORE_H
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPC_HD __host__ __device__ __forceinline__
+#else
+#define MPC_HD inline
+#endif
+
+namespace mpc {
+
+template <class T> struct cx { T re, im; };
+
+template <class T> MPC_HD cx<T> mk(T a, T b) { cx<T> c; c.re = a; c.im = b; return c; }
+template <class T> MPC_HD cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <class T> MPC_HD cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.re - b.re, a.im - b.im); }
+template <class T> MPC_HD cx<T> cmul(cx<T> a, cx<T> b) {
+    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <class T> MPC_HD cx<T> cmulc(cx<T> a, cx<T> b) { /* a * conj(b) */
+    return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+template <class T> MPC_HD cx<T> cscale(cx<T> a, T s) { return mk(a.re * s, a.im * s); }
+/* a * exp(DIR * i pi/2): DIR = -1 -> a * (-i), DIR = +1 -> a * (+i) */
+template <int DIR, class T> MPC_HD cx<T> rot90(cx<T> a) {
+    return DIR < 0 ? mk(a.im, -a.re) : mk(-a.im, a.re);
+}
+
+/* ---- compile-time twiddles: exp(DIR * 2 pi i m / N) ---------------------- */
+constexpr double ce_pi = 3.14159265358979323846264338327950288;
+constexpr double ce_sin_poly(double x) {
+    double x2 = x * x, term = x, sum = x;
+    for (int i = 1; i < 14; ++i) { term *= -x2 / ((2.0 * i) * (2.0 * i + 1.0)); sum += term; }
+    return sum;
+}
+constexpr double ce_cos_poly(double x) {
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int i = 1; i < 14; ++i) { term *= -x2 / ((2.0 * i - 1.0) * (2.0 * i)); sum += term; }
+    return sum;
+}
+/* cos / sin of 2 pi m / N with octant reduction in integer arithmetic */
+constexpr double ce_cos2pi(int m, int N) {
+    m %= N; if (m < 0) m += N;
+    /* reduce to [0, N/2] using cos(2pi - a) = cos a */
+    if (2 * m > N) m = N - m;
+    /* [0, N/4]: direct; (N/4, N/2]: cos a = -cos(pi - a) */
+    if (4 * m > N) return -ce_cos_poly(ce_pi - 2.0 * ce_pi * m / N);
+    return ce_cos_poly(2.0 * ce_pi * m / N);
+}
+constexpr double ce_sin2pi(int m, int N) {
+    m %= N; if (m < 0) m += N;
+    if (2 * m > N) return -ce_sin2pi(N - m, N);
+    if (4 * m > N) return ce_sin_poly(ce_pi - 2.0 * ce_pi * m / N);
+    return ce_sin_poly(2.0 * ce_pi * m / N);
+}
+template <int DIR, class T> MPC_HD cx<T> ctw(int m, int N) {
+    return mk((T)ce_cos2pi(m, N), (T)(DIR * ce_sin2pi(m, N)));
+}
+
+/* ---- small DFT codelets, in place, natural order ------------------------- */
+template <int R, int DIR, class T> struct Dft;
+
+template <int DIR, class T> struct Dft<2, DIR, T> {
+    static MPC_HD void run(cx<T> *x) {
+        const cx<T> a = x[0], b = x[1];
+        x[0] = a + b;
+        x[1] = a - b;
+    }
+};
+
+template <int DIR, class T> struct Dft<3, DIR, T> {
+    static MPC_HD void run(cx<T> *x) {
+        const T h = (T)0.5, s3 = (T)0.86602540378443864676372317075294;
+        const cx<T> t1 = x[1] + x[2];
+        const cx<T> t2 = x[1] - x[2];
+        const cx<T> m = mk(x[0].re - h * t1.re, x[0].im - h * t1.im);
+        /* DIR * i * sqrt(3)/2 * t2 */
+        const cx<T> j = rot90<DIR>(cscale(t2, s3));
+        x[0] = x[0] + t1;
+        x[1] = m + j;
+        x[2] = m - j;
+    }
+};
+
+template <int DIR, class T> struct Dft<4, DIR, T> {
+    static MPC_HD void run(cx<T> *x) {
+        const cx<T> a0 = x[0] + x[2], a1 = x[0] - x[2];
+        const cx<T> a2 = x[1] + x[3], a3 = rot90<DIR>(x[1] - x[3]);
+        x[0] = a0 + a2;
+        x[2] = a0 - a2;
+        x[1] = a1 + a3;
+        x[3] = a1 - a3;
+    }
+};
+
+/* Cooley-Tukey composite R1*R2: n = R2*n1 + n2, k = k1 + R1*k2 */
+template <int R1, int R2, int DIR, class T> struct DftCT {
+    static MPC_HD void run(cx<T> *x) {
+        constexpr int N = R1 * R2;
+        cx<T> z[N];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            cx<T> a[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) a[n1] = x[R2 * n1 + n2];
+            Dft<R1, DIR, T>::run(a);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1)
+                z[n2 * R1 + k1] = (n2 * k1 == 0) ? a[k1] : cmul(a[k1], ctw<DIR, T>(n2 * k1, N));
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            cx<T> b[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) b[n2] = z[n2 * R1 + k1];
+            Dft<R2, DIR, T>::run(b);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) x[k1 + R1 * k2] = b[k2];
+        }
+    }
+};
+template <int DIR, class T> struct Dft<8, DIR, T> {
+    static MPC_HD void run(cx<T> *x) { DftCT<2, 4, DIR, T>::run(x); }
+};
+template <int DIR, class T> struct Dft<12, DIR, T> {
+    static MPC_HD void run(cx<T> *x) { DftCT<4, 3, DIR, T>::run(x); }
+};
+template <int DIR, class T> struct Dft<16, DIR, T> {
+    static MPC_HD void run(cx<T> *x) { DftCT<4, 4, DIR, T>::run(x); }
+};
+
+/*
+ * One Stockham stage.  Ns = product of the radices of earlier stages.
+ * v: the thread's E registers (x[tid + TPL*m]); lds: the line's exchange
+ * buffer, element idx at lds[idx * ls]; tw: N-entry table exp(-2 pi i k/N).
+ */
+template <class T, int N, int E, int R, int Ns, int DIR, bool LAST>
+MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+    constexpr int TPL = N / E;
+    constexpr int B = E / R;
+    static_assert(E % R == 0, "radix must divide elements per thread");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + TPL * b;
+        const int k = j % Ns;
+        cx<T> u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = v[b + r * B];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cx<T> w = tw[k * r * (N / (Ns * R))];
+                if (DIR > 0) w.im = -w.im;
+                u[r] = cmul(u[r], w);
+            }
+        }
+        Dft<R, DIR, T>::run(u);
+        if (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * B] = u[r];
+        } else {
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[(base + r * Ns) * ls] = u[r];
+        }
+    }
+}
+
+template <class T, int N, int E>
+MPC_HD void fft_gather(cx<T> (&v)[E], int tid, const cx<T> *lds, int ls) {
+    constexpr int TPL = N / E;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[(tid + TPL * m) * ls];
+}
+
+/* radix plans: a compile-time list of radices whose product is N */
+template <int... Rs> struct Radices {};
+
+template <int N> struct Plan;
+/* t lines */
+template <> struct Plan<4096> { static constexpr int E = 16; using R = Radices<16, 16, 16>; };
+template <> struct Plan<2048> { static constexpr int E = 16; using R = Radices<16, 16, 8>; };
+template <> struct Plan<1024> { static constexpr int E = 16; using R = Radices<16, 16, 4>; };
+template <> struct Plan<512>  { static constexpr int E = 16; using R = Radices<16, 16, 2>; };
+/* x / y lines */
+template <> struct Plan<256>  { static constexpr int E = 16; using R = Radices<16, 16>; };
+template <> struct Plan<192>  { static constexpr int E = 24; using R = Radices<8, 8, 3>; };
+template <> struct Plan<128>  { static constexpr int E = 16; using R = Radices<16, 8>; };
+template <> struct Plan<64>   { static constexpr int E = 16; using R = Radices<16, 4>; };
+template <> struct Plan<32>   { static constexpr int E = 8;  using R = Radices<8, 4>; };
+template <> struct Plan<16>   { static constexpr int E = 16; using R = Radices<16>; };
+
+} // namespace mpc
+
+#endif

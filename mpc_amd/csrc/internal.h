@@ -1,0 +1,125 @@
+/*
+ * internal.h -- device context and kernel launchers shared by the library's
+ * translation units.  Not part of the public ABI (the headers under include/ are).
+ *
+ * HBM layout of a resident field: A[y][x][t] (t fastest, "line-major"),
+ * complex<T> interleaved, T = double (MPC_F64) or float (MPC_F32).  Every
+ * t-line is contiguous (16 KB FP32 / 32 KB FP64 at Nt = 2048); x- and
+ * y-lines are read as tiles of G consecutive t values (256 B per row).
+ * The reference layout A[t][y][x] is converted at upload/download only.
+ */
+#ifndef MPC_INTERNAL_H
+#define MPC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <map>
+#include <string>
+#include <vector>
+#include "../../include/mpc.h"
+#include "fft_core.h"
+
+namespace mpc {
+
+/* FP64 constants of the plasma model (plasma.c:141-212), evaluated once */
+struct PlasmaK {
+    double hbar_w0;      /* HBAR * w0 */
+    double inv_ef;       /* 1 / EF */
+    double kon1, kon2, kon3, kon4, kon5, kon6;
+    double K1, K2, K3, K4, K5;
+    double w0;
+    double I1e;          /* I1[eV] * EL0 */
+    double Kon;          /* deltat / T0 */
+    double deltat;
+    double rho_nt;
+};
+
+template <class T> struct TArgs {
+    cx<T> *field;
+    const cx<T> *tw;     /* Nt twiddles exp(-2 pi i k / Nt) */
+    const cx<T> *P;      /* pre table, FFT order, includes 1/Nt */
+    const cx<T> *Q;      /* post table (only with out_time) or nullptr */
+    const T *wfft;       /* FFT-order Omega grid */
+    double *rho_block;   /* per-block max(ions1 + 2 ions2), may be nullptr */
+    PlasmaK pk;
+    T dz, gamma, sst_coef; /* sst_coef = gamma2 / w0 */
+    int in_freq, do_nl, out_time, sst_on;
+    long nlines;
+};
+
+template <class T> struct AArgs {
+    cx<T> *field;
+    const cx<T> *tw;     /* n twiddles */
+    const cx<T> *D;      /* n-entry propagator (FFT order), includes 1/n */
+    const cx<T> *M;      /* mirror table M[y*Nx + x] or nullptr */
+    long s_line;         /* element stride along the line */
+    long s_outer;        /* stride between tiles' outer index */
+    long m_line, m_outer;/* mirror index = i*m_line + outer*m_outer */
+    int nt;              /* contiguous t extent */
+};
+
+/* launchers (kernels.hip) */
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, hipStream_t s);
+template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, hipStream_t s);
+template <class T> int launch_to_internal(const double *src, cx<T> *dst, int nx, int ny, int nt, double scale,
+                                          hipStream_t s);
+template <class T> int launch_to_reference(const cx<T> *src, double *dst, int nx, int ny, int nt, double scale,
+                                           hipStream_t s);
+int launch_maxabs(const double *v, long n, double *part, int nb, hipStream_t s);
+template <class T> int launch_scale(cx<T> *f, long n, T s, hipStream_t st);
+template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines,
+                                         double *line_power, double *temp_part, double *spec_part,
+                                         int nblocks, hipStream_t s);
+int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s);
+template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part,
+                                           int nblocks, hipStream_t s);
+template <class T> int launch_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n, hipStream_t s);
+template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const PlasmaK &pk,
+                                    cx<double> *tau_F, hipStream_t s);
+int launch_adk(const double *E, long n, const double *kon, double *W1, double *W2, hipStream_t s);
+
+bool tline_supported(int nt);
+bool axis_supported(int n);
+int axis_tile(int precision);   /* G: t values per axis tile */
+
+} // namespace mpc
+
+/* the context behind mpc_ctx* */
+struct mpc_ctx_s {
+    mpc_params p;
+    std::vector<double> betas, wr, w, r;
+    int prec = MPC_F64;
+    int nx = 0, ny = 0, nt = 0;
+    long nxy = 0, N = 0;
+    size_t esz = 16;                 /* bytes per complex element */
+    hipStream_t stream = nullptr;
+    int device = 0;
+    void *field = nullptr;           /* resident field, internal layout */
+    void *aux1 = nullptr, *aux2 = nullptr; /* adaptive C1/C2 (lazily) */
+    double *staging = nullptr;       /* FP64 reference-layout staging (N complex) */
+    void *tw_t = nullptr, *tw_x = nullptr;   /* twiddles */
+    void *wfft = nullptr;            /* FFT-order Omega, precision T */
+    void *mirror = nullptr;          /* mirror table [y][x] */
+    double *rho_block = nullptr;     /* per t-line block */
+    long rho_nblocks = 0;
+    double *diag_part = nullptr;     /* diagnostics partials */
+    double *diag_line = nullptr;     /* per-line power */
+    double *diag_out = nullptr;      /* reduced temp/spec */
+    double *err_part = nullptr;
+    int diag_nblocks = 0;
+    mpc::PlasmaK pk{};
+    std::map<std::string, void *> tables;   /* table cache keyed by kind+coef */
+    int domain_freq = 0;             /* resident field is F_t(A) (unnormalised) */
+    /* the resident field holds A / ascale; ascale is a power of two chosen
+     * at upload so |A / ascale| <= 1 (photon-flux amplitudes reach 1e18 and
+     * their SST products would overflow FP32).  Nonlinear coefficients are
+     * multiplied by ascale^2 instead. */
+    double ascale = 1.0;
+    double rho_peak = 0.0;
+    /* timing */
+    int timing = 0;
+    hipEvent_t ev[8] = {};
+    double kms[3] = {0, 0, 0};
+    long kcount[3] = {0, 0, 0};
+};
+
+#endif

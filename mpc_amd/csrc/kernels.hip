@@ -1,0 +1,683 @@
+/*
+ * kernels.hip -- the gfx950 kernels of the split-step propagator.
+ *
+ * Sweeps over the resident field (layout A[y][x][t], t fastest):
+ *   k_tline  : one workgroup owns whole t-lines.  [t-FFT] x P(omega) ->
+ *              t-IFFT of A and of -i omega A (spectral d/dt) -> Kerr +
+ *              self-steepening + plasma (ADK + avalanche, reverse-t affine
+ *              scan in FP64) -> A exp(dz A_nl) -> t-FFT [x Q -> t-IFFT].
+ *              Replaces propagation.c:164-327 and plasma.c:118-246 (which
+ *              the reference runs on the host) and the t part of linearstep
+ *              (propagation.c:11-30, 67-76).
+ *   k_axis   : FFT along x or y of a tile of G consecutive t values,
+ *              x D(k), inverse FFT [x mirror].  Replaces the x/y parts of
+ *              linearstep (propagation.c:31-57, 77-100); the diffraction
+ *              operator is separable, so x and y each take one sweep.
+ * plus layout transposes, diagnostics reductions, the adaptive-step error
+ * and Richardson sweeps, and the plasma/ADK kernels behind the legacy
+ * ioniz()/ADK() entry points.
+ */
+#include "internal.h"
+#include <cmath>
+#include <cstdio>
+
+namespace mpc {
+
+/* ---------------------------------------------------------------------- */
+/* FFT driver: unrolls Plan<N>::R into stages with LDS exchange between    */
+/* ---------------------------------------------------------------------- */
+template <class T, int N, int E, int DIR, int Ns, int R, int... Rest>
+struct DevFft {
+    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
+                                               const cx<T> *tw) {
+        constexpr bool last = sizeof...(Rest) == 0;
+        fft_stage<T, N, E, R, Ns, DIR, last>(v, tid, lds, ls, tw);
+        if constexpr (!last) {
+            __syncthreads();
+            fft_gather<T, N, E>(v, tid, lds, ls);
+            __syncthreads();
+            DevFft<T, N, E, DIR, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
+        }
+    }
+};
+template <class T, int N, int DIR, class Rad> struct DevFftP;
+template <class T, int N, int DIR, int... Rs> struct DevFftP<T, N, DIR, Radices<Rs...>> {
+    static __device__ __forceinline__ void run(cx<T> (&v)[Plan<N>::E], int tid, cx<T> *lds,
+                                               int ls, const cx<T> *tw) {
+        DevFft<T, N, Plan<N>::E, DIR, 1, Rs...>::run(v, tid, lds, ls, tw);
+    }
+};
+template <class T, int N, int DIR>
+__device__ __forceinline__ void line_fft(cx<T> (&v)[Plan<N>::E], int tid, cx<T> *lds, int ls,
+                                         const cx<T> *tw) {
+    DevFftP<T, N, DIR, typename Plan<N>::R>::run(v, tid, lds, ls, tw);
+}
+
+/* ---------------------------------------------------------------------- */
+/* plasma model pieces (FP64 in both precisions)                           */
+/* ---------------------------------------------------------------------- */
+struct PEl {
+    double b, a, c;      /* W1*Kon, avalanche*dt, W2*Kon */
+    double sig, tau;
+};
+
+/* ADK rate Kon_a (Kon_b/E)^Kon_c exp(-Kon_b/(3E)) in log form; exactly 0
+ * where the exponential underflows in double precision (plasma.c:55-56). */
+__device__ __forceinline__ double adk_rate(double ka, double kb, double kc, double Ea) {
+    if (ka == 0.0) return 0.0;
+    const double e = kb / (3.0 * Ea);
+    if (e > 760.0) return 0.0;
+    const double w = ka * exp(kc * log(kb / Ea) - e);
+    return (w != w) ? 0.0 : w;
+}
+
+/* per-element plasma coefficients from |A|^2 (plasma.c:133-212, corrected
+ * arithmetic: quiver velocity at w0, avalanche rate in SI units) */
+__device__ __forceinline__ PEl plasma_el(double a2, const PlasmaK &k, bool rates) {
+    PEl r;
+    double I = a2 * k.hbar_w0;
+    if (I == 0.0) I = 1e-30;
+    const double E0 = sqrt(2.0 * 377.0 * I / 1.0) * 0.89;
+    const double Ea = fabs(E0) * k.inv_ef;
+    const double ve = sqrt(k.K1 * (E0 * E0) / (k.K2 * (k.w0 * k.w0)));
+    const double tc = 1.0 / (k.K3 * ve);
+    r.tau = tc;
+    r.sig = k.K4 * tc / (1.0 + k.K5 * (tc * tc));
+    if (rates) {
+        const double W1 = adk_rate(k.kon1, k.kon3, k.kon5, Ea);
+        const double W2 = adk_rate(k.kon2, k.kon4, k.kon6, Ea);
+        const double Wava = r.sig * I / k.I1e;
+        r.b = W1 * k.Kon;
+        r.a = Wava * k.deltat;
+        r.c = W2 * k.Kon;
+    } else {
+        r.b = r.a = r.c = 0.0;
+    }
+    return r;
+}
+
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+
+/*
+ * Reverse-t affine scan over one line.  Element k carries the map
+ * f_k(R) = M[k] R + C[k]; the result R_k = f_k(R_{k+1}) with R_NT = 0 is
+ * written over M[k].  Thread tid owns the consecutive segment
+ * [tid*E, tid*E + E); segment maps are composed serially, combined across
+ * lanes with a shuffle-based suffix scan and across wavefronts through LDS.
+ */
+template <int NT>
+__device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const double *C, double *scr) {
+    constexpr int E = Plan<NT>::E, TPL = NT / E;
+    constexpr int W = TPL < 64 ? TPL : 64;
+    constexpr int NW = (TPL + 63) / 64;
+    const int lo = tid * E;
+    double gm = 1.0, gc = 0.0;
+#pragma unroll
+    for (int i = E - 1; i >= 0; --i) {
+        const int k = pad16(lo + i);
+        const double mk = M[k], ck = C[k];
+        gc = mk * gc + ck;
+        gm = mk * gm;
+    }
+    const int lane = tid % W;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        const double om = __shfl_down(gm, d, W);
+        const double oc = __shfl_down(gc, d, W);
+        if (lane + d < W) {
+            gc = gm * oc + gc;
+            gm = gm * om;
+        }
+    }
+    if constexpr (NW > 1) {
+        const int wv = tid / 64;
+        if (lane == 0) { scr[2 * wv] = gm; scr[2 * wv + 1] = gc; }
+        __syncthreads();
+        for (int w2 = wv + 1; w2 < NW; ++w2) {
+            const double om = scr[2 * w2], oc = scr[2 * w2 + 1];
+            gc = gm * oc + gc;
+            gm = gm * om;
+        }
+    }
+    scr[2 * NW + tid] = gc;
+    __syncthreads();
+    double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
+#pragma unroll
+    for (int i = E - 1; i >= 0; --i) {
+        const int k = pad16(lo + i);
+        R = M[k] * R + C[k];
+        M[k] = R;
+    }
+    __syncthreads();
+}
+
+template <int NT> struct PlasmaLds {
+    static constexpr int E = Plan<NT>::E, TPL = NT / E, NW = (TPL + 63) / 64;
+    static constexpr int ARR = NT + NT / 16;           /* padded doubles per array */
+    static constexpr int SCR = 2 * NW + TPL;
+    static constexpr int BYTES = (2 * ARR + SCR) * 8;
+};
+
+/*
+ * Density of one line (plasma.c:218-241, the consistent two-species form):
+ *   R_k     = (R_{k+1} + b_k)(1 - a_k) + a_k,                 R_NT = 0
+ *   u_k     = 1 - exp(-R_k)      (evaluated as -expm1(-R_k))
+ *   ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1},            ions2_NT = 0
+ *   ions1_k = u_k - ions2_k ;  F_k = (ions1_k + 2 ions2_k) sigma_k
+ * Element k = tid + TPL*m of the line is held by thread tid in slot m.
+ * sink(m, F, tau) receives the result per element.
+ */
+template <class T, int NT, class Sink>
+__device__ __forceinline__ void plasma_line(const cx<T> (&A)[Plan<NT>::E], int tid, double *lds,
+                                            const PlasmaK &pk, double &rho_max, Sink sink) {
+    using L = PlasmaLds<NT>;
+    constexpr int E = L::E, TPL = L::TPL;
+    double *Lm = lds, *Lc = lds + L::ARR, *scr = lds + 2 * L::ARR;
+    double cc[E];
+    int anyc = 0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = tid + TPL * m;
+        const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
+        const PEl e = plasma_el(a2, pk, true);
+        Lm[pad16(k)] = 1.0 - e.a;
+        Lc[pad16(k)] = e.b * (1.0 - e.a) + e.a;
+        cc[m] = e.c;
+        anyc |= (e.c != 0.0);
+    }
+    __syncthreads();
+    affine_suffix_scan<NT>(tid, Lm, Lc, scr);         /* Lm <- R */
+    const int block_anyc = __syncthreads_or(anyc);
+    double u[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) u[m] = -expm1(-Lm[pad16(tid + TPL * m)]);
+    if (block_anyc) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int k = tid + TPL * m;
+            const double un = (k + 1 < NT) ? -expm1(-Lm[pad16(k + 1)]) : 0.0;
+            Lc[pad16(k)] = 1.0 - cc[m];   /* M' */
+            cc[m] = cc[m] * un;           /* C' */
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) Lm[pad16(tid + TPL * m)] = cc[m];
+        __syncthreads();
+        affine_suffix_scan<NT>(tid, Lc, Lm, scr);     /* Lc <- ions2 */
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = tid + TPL * m;
+        const double i2 = block_anyc ? Lc[pad16(k)] : 0.0;
+        const double i1 = u[m] - i2;
+        const double fr = i1 + 2.0 * i2;
+        const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
+        const PEl e = plasma_el(a2, pk, false);
+        rho_max = fmax(rho_max, fr);
+        sink(m, fr * e.sig, e.tau);
+    }
+    __syncthreads();
+}
+
+/* ---------------------------------------------------------------------- */
+/* t-line kernel                                                           */
+/* ---------------------------------------------------------------------- */
+template <class T, int NT, bool PLASMA> struct TCfg {
+    static constexpr int E = Plan<NT>::E, TPL = NT / E;
+    static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
+    static constexpr int THREADS = LPB * TPL;
+    static constexpr int FFT_BYTES = NT * (int)sizeof(cx<T>);
+    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<NT>::BYTES : 0;
+    static constexpr int LINE_BYTES = ((FFT_BYTES > PL_BYTES ? FFT_BYTES : PL_BYTES) + 15) / 16 * 16;
+};
+
+template <class T> __device__ __forceinline__ void sincos_t(T x, T *s, T *c);
+template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) { sincos(x, s, c); }
+template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+template <class T> __device__ __forceinline__ T exp_t(T x);
+template <> __device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
+
+template <class T, int NT, bool PLASMA>
+__global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<T> a) {
+    using C = TCfg<T, NT, PLASMA>;
+    constexpr int E = C::E, TPL = C::TPL;
+    __shared__ __align__(16) unsigned char smem[C::LPB * C::LINE_BYTES];
+    const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
+    const long line = (long)blockIdx.x * C::LPB + lslot;
+    unsigned char *lraw = smem + lslot * C::LINE_BYTES;
+    cx<T> *L = reinterpret_cast<cx<T> *>(lraw);
+    cx<T> *g = a.field + line * NT;
+
+    cx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = g[tid + TPL * m];
+    if (!a.in_freq) line_fft<T, NT, -1>(v, tid, L, 1, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
+
+    if (a.do_nl) {
+        cx<T> d[E];
+        if (a.sst_on) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const T wk = a.wfft[tid + TPL * m];
+                d[m] = mk(wk * v[m].im, -wk * v[m].re);   /* -i omega A(omega) */
+            }
+            line_fft<T, NT, +1>(d, tid, L, 1, a.tw);      /* dA */
+        }
+        line_fft<T, NT, +1>(v, tid, L, 1, a.tw);          /* A  */
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
+            if (a.sst_on) {
+                /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232) */
+                const cx<T> t1 = cmulc(d[m], v[m]);
+                const cx<T> t2 = cmulc(v[m], d[m]);
+                const T sre = a.sst_coef * ((T)2 * t1.re + t2.re);
+                const T sim = a.sst_coef * ((T)2 * t1.im + t2.im);
+                d[m] = mk(-sre, a.gamma * a2 - sim);
+            } else {
+                d[m] = mk((T)0, a.gamma * a2);
+            }
+        }
+        if constexpr (PLASMA) {
+            double rmax = 0.0;
+            const double half_rho = 0.5 * a.pk.rho_nt;
+            plasma_line<T, NT>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
+                               [&](int m, double F, double tau) {
+                                   /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
+                                   const double z = half_rho * F;
+                                   d[m].re -= (T)z;
+                                   d[m].im -= (T)(a.pk.w0 * tau * z);
+                               });
+            if (a.rho_block) {
+                /* block max of the ionisation fraction */
+                __shared__ double red[32];
+                double r = rmax;
+                for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_down(r, o));
+                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    double mx = 0.0;
+                    for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mx = fmax(mx, red[i]);
+                    a.rho_block[blockIdx.x] = mx;
+                }
+            }
+        }
+        /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T er = exp_t<T>(a.dz * d[m].re);
+            T sn, cs;
+            sincos_t<T>(a.dz * d[m].im, &sn, &cs);
+            const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
+            v[m] = mk(er * z.re, er * z.im);
+        }
+        line_fft<T, NT, -1>(v, tid, L, 1, a.tw);
+    }
+    if (a.Q) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
+    }
+    if (a.out_time) line_fft<T, NT, +1>(v, tid, L, 1, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
+}
+
+bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
+
+template <class T, int NT, bool PL>
+static int tline_go(const TArgs<T> &a, hipStream_t s) {
+    using C = TCfg<T, NT, PL>;
+    if (a.nlines % C::LPB) return MPC_EINVAL;
+    const long nb = a.nlines / C::LPB;
+    hipLaunchKernelGGL((k_tline<T, NT, PL>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, hipStream_t s) {
+    switch (nt) {
+    case 512:  return plasma ? tline_go<T, 512, true>(a, s)  : tline_go<T, 512, false>(a, s);
+    case 1024: return plasma ? tline_go<T, 1024, true>(a, s) : tline_go<T, 1024, false>(a, s);
+    case 2048: return plasma ? tline_go<T, 2048, true>(a, s) : tline_go<T, 2048, false>(a, s);
+    case 4096: return plasma ? tline_go<T, 4096, true>(a, s) : tline_go<T, 4096, false>(a, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_tline<double>(const TArgs<double> &, int, bool, hipStream_t);
+template int launch_tline<float>(const TArgs<float> &, int, bool, hipStream_t);
+
+/* t-line kernel block geometry, for callers that size per-block buffers */
+int tline_lines_per_block(int nt, bool plasma, int prec) {
+    (void)plasma; (void)prec;
+    switch (nt) {
+    case 512: return TCfg<double, 512, true>::LPB;
+    case 1024: return TCfg<double, 1024, true>::LPB;
+    case 2048: return TCfg<double, 2048, true>::LPB;
+    case 4096: return TCfg<double, 4096, true>::LPB;
+    default: return 1;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* x / y line kernel                                                       */
+/* ---------------------------------------------------------------------- */
+template <class T> struct ATile { static constexpr int G = 256 / (int)sizeof(cx<T>); };  /* 256 B rows */
+int axis_tile(int precision) { return precision == MPC_F32 ? ATile<float>::G : ATile<double>::G; }
+
+template <class T, int N> struct ACfg {
+    static constexpr int E = Plan<N>::E, TPL = N / E, G = ATile<T>::G;
+    static constexpr int THREADS = G * TPL;
+};
+
+template <class T, int N>
+__global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
+    using C = ACfg<T, N>;
+    constexpr int E = C::E, TPL = C::TPL, G = C::G;
+    __shared__ __align__(16) cx<T> lds[N * G];
+    const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
+    const long outer = blockIdx.y;
+    const long t0 = (long)blockIdx.x * G;
+    cx<T> *base = a.field + outer * a.s_outer + t0 + gcol;
+    cx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = base[(long)(tid + TPL * m) * a.s_line];
+    line_fft<T, N, -1>(v, tid, lds + gcol, G, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.D[tid + TPL * m]);
+    line_fft<T, N, +1>(v, tid, lds + gcol, G, a.tw);
+    if (a.M) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
+}
+
+bool axis_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+
+template <class T, int N> static int axis_go(const AArgs<T> &a, int outer, hipStream_t s) {
+    using C = ACfg<T, N>;
+    if (a.nt % C::G) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_axis<T, N>), dim3((unsigned)(a.nt / C::G), (unsigned)outer), dim3(C::THREADS), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, hipStream_t s) {
+    switch (n) {
+    case 64:  return axis_go<T, 64>(a, outer, s);
+    case 128: return axis_go<T, 128>(a, outer, s);
+    case 192: return axis_go<T, 192>(a, outer, s);
+    case 256: return axis_go<T, 256>(a, outer, s);
+    case 512: return axis_go<T, 512>(a, outer, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_axis<double>(const AArgs<double> &, int, int, hipStream_t);
+template int launch_axis<float>(const AArgs<float> &, int, int, hipStream_t);
+
+/* ---------------------------------------------------------------------- */
+/* layout conversion: reference A[t][y][x] (FP64) <-> internal A[y][x][t]  */
+/* ---------------------------------------------------------------------- */
+template <class T>
+__global__ void __launch_bounds__(256) k_to_internal(const double *__restrict__ src, cx<T> *__restrict__ dst,
+                                                     int nx, int ny, int nt, double scale) {
+    __shared__ double2 tile[32][33];
+    const int x0 = blockIdx.x * 32, t0 = blockIdx.y * 32, y = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   /* 32 x 8 */
+    for (int i = ty; i < 32; i += 8) {
+        const long s = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
+        tile[i][tx] = reinterpret_cast<const double2 *>(src)[s];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const double2 e = tile[tx][i];                           /* t = t0+tx, x = x0+i */
+        const long d = ((long)y * nx + x0 + i) * nt + t0 + tx;
+        dst[d] = mk((T)(e.x * scale), (T)(e.y * scale));
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_to_reference(const cx<T> *__restrict__ src, double *__restrict__ dst,
+                                                      int nx, int ny, int nt, double scale) {
+    __shared__ double2 tile[32][33];
+    const int x0 = blockIdx.x * 32, t0 = blockIdx.y * 32, y = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const cx<T> e = src[((long)y * nx + x0 + i) * nt + t0 + tx];   /* x = x0+i, t = t0+tx */
+        tile[tx][i] = make_double2((double)e.re * scale, (double)e.im * scale);
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const long d = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
+        reinterpret_cast<double2 *>(dst)[d] = tile[i][tx];
+    }
+}
+template <class T> int launch_to_internal(const double *src, cx<T> *dst, int nx, int ny, int nt, double scale, hipStream_t s) {
+    if (nx % 32 || nt % 32) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_to_internal<T>), dim3(nx / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T> int launch_to_reference(const cx<T> *src, double *dst, int nx, int ny, int nt, double scale, hipStream_t s) {
+    if (nx % 32 || nt % 32) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_to_reference<T>), dim3(nx / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_to_internal<double>(const double *, cx<double> *, int, int, int, double, hipStream_t);
+template int launch_to_internal<float>(const double *, cx<float> *, int, int, int, double, hipStream_t);
+template int launch_to_reference<double>(const cx<double> *, double *, int, int, int, double, hipStream_t);
+template int launch_to_reference<float>(const cx<float> *, double *, int, int, int, double, hipStream_t);
+
+/* per-block max of |re|, |im| over n doubles (host reduces the 1024 partials) */
+__global__ void __launch_bounds__(256) k_maxabs(const double *v, long n, double *part) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) m = fmax(m, fabs(v[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+int launch_maxabs(const double *v, long n, double *part, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(256), 0, s, v, n, part);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+template <class T> __global__ void k_scale(cx<T> *f, long n, T s) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        f[i] = mk(f[i].re * s, f[i].im * s);
+}
+template <class T> int launch_scale(cx<T> *f, long n, T s, hipStream_t st) {
+    hipLaunchKernelGGL((k_scale<T>), dim3(2048), dim3(256), 0, st, f, n, s);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_scale<double>(cx<double> *, long, double, hipStream_t);
+template int launch_scale<float>(cx<float> *, long, float, hipStream_t);
+
+/* ---------------------------------------------------------------------- */
+/* diagnostics: per-line power, and per-block partial temp[t] / spec[t]    */
+/* ---------------------------------------------------------------------- */
+template <class T, int NT>
+__global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> *tw, long nlines,
+                                                    double *line_power, double *temp_part, double *spec_part) {
+    constexpr int E = Plan<NT>::E, TPL = NT / E, LPB = (256 / TPL) > 0 ? 256 / TPL : 1;
+    __shared__ __align__(16) cx<T> lds[LPB * NT];
+    __shared__ double accT[NT], accS[NT];
+    __shared__ double lp[LPB][TPL / 64 + 1];
+    const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
+    for (int i = threadIdx.x; i < NT; i += blockDim.x) { accT[i] = 0.0; accS[i] = 0.0; }
+    double at[E], as[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) { at[m] = 0.0; as[m] = 0.0; }
+    /* a fixed set of line groups per block: deterministic order */
+    const long ngroups = nlines / LPB;
+    for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const long line = grp * LPB + lslot;
+        const cx<T> *g = f + line * NT;
+        cx<T> v[E];
+        double pw = 0.0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            v[m] = g[tid + TPL * m];
+            const double p = (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
+            at[m] += p;
+            pw += p;
+        }
+        /* line power: reduce over the TPL threads of the line */
+        for (int o = (TPL < 64 ? TPL : 64) / 2; o > 0; o >>= 1) pw += __shfl_down(pw, o, TPL < 64 ? TPL : 64);
+        if (tid % 64 == 0) lp[lslot][tid / 64] = pw;
+        line_fft<T, NT, -1>(v, tid, lds + lslot * NT, 1, tw);
+#pragma unroll
+        for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int i = 0; i < (TPL + 63) / 64; ++i) s += lp[lslot][i];
+            line_power[line] = s;
+        }
+        __syncthreads();
+    }
+    /* combine the LPB line slots in slot order */
+    for (int sl = 0; sl < LPB; ++sl) {
+        if (lslot == sl) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) { accT[tid + TPL * m] += at[m]; accS[tid + TPL * m] += as[m]; }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < NT; i += blockDim.x) {
+        temp_part[(long)blockIdx.x * NT + i] = accT[i];
+        spec_part[(long)blockIdx.x * NT + i] = accS[i];
+    }
+}
+template <class T, int NT>
+static int diag_go(const cx<T> *f, const cx<T> *tw, long nlines, double *lp, double *tp, double *sp, int nb, hipStream_t s) {
+    constexpr int TPL = NT / Plan<NT>::E, LPB = (256 / TPL) > 0 ? 256 / TPL : 1;
+    if (nlines % LPB) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_diag_lines<T, NT>), dim3(nb), dim3(LPB * TPL), 0, s, f, tw, nlines, lp, tp, sp);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines, double *lp,
+                                         double *tp, double *sp, int nb, hipStream_t s) {
+    switch (nt) {
+    case 512: return diag_go<T, 512>(f, tw, nlines, lp, tp, sp, nb, s);
+    case 1024: return diag_go<T, 1024>(f, tw, nlines, lp, tp, sp, nb, s);
+    case 2048: return diag_go<T, 2048>(f, tw, nlines, lp, tp, sp, nb, s);
+    case 4096: return diag_go<T, 4096>(f, tw, nlines, lp, tp, sp, nb, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_diag_lines<double>(const cx<double> *, const cx<double> *, int, long, double *, double *, double *, int, hipStream_t);
+template int launch_diag_lines<float>(const cx<float> *, const cx<float> *, int, long, double *, double *, double *, int, hipStream_t);
+
+/* out[c] = sum_r part[r][c], in row order (deterministic) */
+__global__ void k_reduce_cols(const double *part, int nrows, int ncols, double *out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (int r = 0; r < nrows; ++r) s += part[(long)r * ncols + c];
+    out[c] = s;
+}
+int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 255) / 256), dim3(256), 0, s, part, nrows, ncols, out);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+/* ---------------------------------------------------------------------- */
+/* adaptive step: error partials and Richardson (propagation.c:625-631,    */
+/* MPC.c:377-381)                                                          */
+/* ---------------------------------------------------------------------- */
+template <class T>
+__global__ void __launch_bounds__(256) k_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part) {
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double dr = (double)c2[i].re - (double)c1[i].re, di = (double)c2[i].im - (double)c1[i].im;
+        a += dr * dr + di * di;
+        b += (double)c2[i].re * c2[i].re + (double)c2[i].im * c2[i].im;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1[0]; part[2 * blockIdx.x + 1] = s2[0]; }
+}
+template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part, int nb, hipStream_t s) {
+    hipLaunchKernelGGL((k_err_partials<T>), dim3(nb), dim3(256), 0, s, c1, c2, n, part);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_err_partials<double>(const cx<double> *, const cx<double> *, long, double *, int, hipStream_t);
+template int launch_err_partials<float>(const cx<float> *, const cx<float> *, long, double *, int, hipStream_t);
+
+template <class T> __global__ void k_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double r = (4.0 / 3.0) * (double)c2[i].re - (1.0 / 3.0) * (double)c1[i].re;
+        const double m = (4.0 / 3.0) * (double)c2[i].im - (1.0 / 3.0) * (double)c1[i].im;
+        a[i] = mk((T)r, (T)m);
+    }
+}
+template <class T> int launch_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n, hipStream_t s) {
+    hipLaunchKernelGGL((k_richardson<T>), dim3(2048), dim3(256), 0, s, a, c1, c2, n);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_richardson<double>(cx<double> *, const cx<double> *, const cx<double> *, long, hipStream_t);
+template int launch_richardson<float>(cx<float> *, const cx<float> *, const cx<float> *, long, hipStream_t);
+
+/* ---------------------------------------------------------------------- */
+/* legacy ioniz(): tau_c and F per element (internal layout outputs)       */
+/* ---------------------------------------------------------------------- */
+template <class T, int NT>
+__global__ void __launch_bounds__((TCfg<T, NT, true>::THREADS)) k_ioniz(const cx<T> *f, long nlines, PlasmaK pk,
+                                                                     cx<double> *out) {
+    using C = TCfg<T, NT, true>;
+    constexpr int E = C::E, TPL = C::TPL;
+    __shared__ __align__(16) unsigned char smem[C::LPB * C::LINE_BYTES];
+    const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
+    const long line = (long)blockIdx.x * C::LPB + lslot;
+    cx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = f[line * NT + tid + TPL * m];
+    double rmax = 0.0;
+    plasma_line<T, NT>(v, tid, reinterpret_cast<double *>(smem + lslot * C::LINE_BYTES), pk, rmax,
+                       [&](int m, double Fv, double tv) {
+                           out[line * NT + tid + TPL * m] = mk(tv, Fv);   /* (tau_c, F) */
+                       });
+}
+template <class T, int NT> static int ioniz_go(const cx<T> *f, long nlines, const PlasmaK &pk, cx<double> *out, hipStream_t s) {
+    using C = TCfg<T, NT, true>;
+    if (nlines % C::LPB) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_ioniz<T, NT>), dim3((unsigned)(nlines / C::LPB)), dim3(C::THREADS), 0, s, f, nlines, pk, out);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const PlasmaK &pk, cx<double> *out, hipStream_t s) {
+    switch (nt) {
+    case 512: return ioniz_go<T, 512>(f, nlines, pk, out, s);
+    case 1024: return ioniz_go<T, 1024>(f, nlines, pk, out, s);
+    case 2048: return ioniz_go<T, 2048>(f, nlines, pk, out, s);
+    case 4096: return ioniz_go<T, 4096>(f, nlines, pk, out, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_ioniz<double>(const cx<double> *, int, long, const PlasmaK &, cx<double> *, hipStream_t);
+template int launch_ioniz<float>(const cx<float> *, int, long, const PlasmaK &, cx<double> *, hipStream_t);
+
+/* legacy ADK(): rates from field amplitudes in atomic units (plasma.c:43-58) */
+__global__ void k_adk(const double *E, long n, double k0, double k1, double k2, double k3, double k4,
+                      double k5, double *W1, double *W2) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        /* no NaN filtering here: the reference does that in ioniz (plasma.c:184-192) */
+        W1[i] = k0 * pow(k2 / E[i], k4) * exp(-k2 / (3.0 * E[i]));
+        W2[i] = k1 * pow(k3 / E[i], k5) * exp(-k3 / (3.0 * E[i]));
+    }
+}
+int launch_adk(const double *E, long n, const double *kon, double *W1, double *W2, hipStream_t s) {
+    hipLaunchKernelGGL(k_adk, dim3(1024), dim3(256), 0, s, E, n, kon[0], kon[1], kon[2], kon[3], kon[4], kon[5], W1, W2);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+} // namespace mpc

@@ -1,0 +1,83 @@
+// CPU emulation of the device line FFT (mpc_amd/csrc/fft_core.h): runs the
+// per-thread stage code in a loop over threads between "barriers" and
+// compares every plan size against a brute-force DFT in long double.
+// Prints the max relative error per (N, precision, direction); exit 1 on failure.
+#include "../../mpc_amd/csrc/fft_core.h"
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+using namespace mpc;
+
+template <class T, int N, int E, int DIR, int Ns, int R, int... Rest>
+struct HostRun {
+    static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const cx<T> *tw) {
+        constexpr int TPL = N / E;
+        constexpr bool last = sizeof...(Rest) == 0;
+        for (int tid = 0; tid < TPL; ++tid) {
+            cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
+            fft_stage<T, N, E, R, Ns, DIR, last>(v, tid, lds.data(), 1, tw);
+        }
+        if constexpr (!last) {
+            for (int tid = 0; tid < TPL; ++tid) {
+                cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
+                fft_gather<T, N, E>(v, tid, lds.data(), 1);
+            }
+            HostRun<T, N, E, DIR, Ns * R, Rest...>::run(V, lds, tw);
+        }
+    }
+};
+template <class T, int N, int DIR, class Rad> struct Unpack;
+template <class T, int N, int DIR, int... Rs> struct Unpack<T, N, DIR, Radices<Rs...>> {
+    static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const cx<T> *tw) {
+        HostRun<T, N, Plan<N>::E, DIR, 1, Rs...>::run(V, lds, tw);
+    }
+};
+
+template <class T, int N, int DIR> double check(unsigned seed) {
+    constexpr int E = Plan<N>::E, TPL = N / E;
+    std::vector<cx<T>> tw(N), lds(N), V((size_t)TPL * E);
+    for (int k = 0; k < N; ++k) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+        tw[k] = mk((T)cosl(a), (T)sinl(a));
+    }
+    std::vector<long double> xr(N), xi(N);
+    srand(seed);
+    for (int i = 0; i < N; ++i) { xr[i] = (T)(rand() / (double)RAND_MAX - 0.5); xi[i] = (T)(rand() / (double)RAND_MAX - 0.5); }
+    for (int tid = 0; tid < TPL; ++tid)
+        for (int m = 0; m < E; ++m) V[(size_t)tid * E + m] = mk((T)xr[tid + TPL * m], (T)xi[tid + TPL * m]);
+    Unpack<T, N, DIR, typename Plan<N>::R>::run(V, lds, tw.data());
+    double num = 0, den = 0;
+    for (int k = 0; k < N; ++k) {
+        long double sr = 0, si = 0;
+        for (int j = 0; j < N; ++j) {
+            long double a = DIR * 2.0L * 3.14159265358979323846264338327950288L * (long double)(((long)j * k) % N) / N;
+            sr += xr[j] * cosl(a) - xi[j] * sinl(a);
+            si += xr[j] * sinl(a) + xi[j] * cosl(a);
+        }
+        const int tid = k % TPL, m = k / TPL;
+        const cx<T> g = V[(size_t)tid * E + m];
+        num += (double)((g.re - sr) * (g.re - sr) + (g.im - si) * (g.im - si));
+        den += (double)(sr * sr + si * si);
+    }
+    return std::sqrt(num / den);
+}
+
+template <int N> int run_n() {
+    int bad = 0;
+    double e1 = check<double, N, -1>(N), e2 = check<double, N, +1>(N + 1);
+    double e3 = check<float, N, -1>(N + 2), e4 = check<float, N, +1>(N + 3);
+    printf("N=%5d f64 fwd %.3e inv %.3e  f32 fwd %.3e inv %.3e\n", N, e1, e2, e3, e4);
+    if (e1 > 1e-14 || e2 > 1e-14 || e3 > 2e-6 || e4 > 2e-6) bad = 1;
+    return bad;
+}
+
+int main() {
+    int bad = 0;
+    bad |= run_n<16>(); bad |= run_n<32>(); bad |= run_n<64>(); bad |= run_n<128>();
+    bad |= run_n<192>(); bad |= run_n<256>(); bad |= run_n<512>(); bad |= run_n<1024>();
+    bad |= run_n<2048>(); bad |= run_n<4096>();
+    printf(bad ? "FAIL\n" : "OK\n");
+    return bad;
+}

@@ -1,0 +1,176 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Bar: FP64 field rel L2 <= 1e-10 against the oracle on identical seeded
+inputs; FP32 field rel L2 <= 1e-4 (the precision-form target).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mpc_amd import Propagator, configs, make_inputs
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-4
+
+
+def _case(nx, nt, **kw):
+    inp = make_inputs(configs.small(nx, nt, **kw))
+    return inp, inp.calibrated()
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+@pytest.mark.parametrize("nx,nt,plasm,sst", [(64, 512, 1, 1), (64, 512, 0, 0), (128, 1024, 1, 1), (64, 2048, 1, 1)])
+def test_run_matches_oracle(gpu_lib, prec, tol, nx, nt, plasm, sst):
+    print(f"case run nx={nx} nt={nt} plasm={plasm} sst={sst} prec={prec}", flush=True)
+    inp, A0 = _case(nx, nt, plasm=plasm, sst_on=sst, steps_per_pass=3, passes=3)
+    nsteps = 7  # crosses two mirrors
+    ref, rho_ref = O.run_fixed(inp, A0, 0, nsteps)
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, nsteps)
+        got = p.download()
+        d = p.diagnostics()
+    err = O.rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e} rho_gpu={d['rho_peak']:.6e}", flush=True)
+    assert err <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+def test_single_step_matches_oracle(gpu_lib, prec, tol):
+    print(f"case single step prec={prec}", flush=True)
+    inp, A0 = _case(32, 512, plasm=1, sst_on=1, mirror_R=0.0)
+    L = O.oracle()
+    ref = A0.copy()
+    p_o = O.params_from(inp)
+    sm = C.c_int(1)
+    pe = C.c_double(inp.params.puls_e)
+    z = 0.3 * inp.params.Cav
+    bp1, bp2, y1, y2 = np.zeros(32), np.zeros(32), np.zeros(4), np.zeros(4)
+    b1, b2 = C.c_double(0), C.c_double(0)
+    L.oracle_ssfmprop(O.vp(ref), C.byref(p_o), inp.params.dzz, C.byref(sm), z, C.byref(pe), O.dp(bp1), O.dp(bp2),
+                      O.dp(y1), O.dp(y2), C.byref(b1), C.byref(b2), 0)
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        s = p.step(inp.params.dzz, z, 1)
+        got = p.download()
+    err = O.rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert s == sm.value
+    assert err <= tol
+
+
+def test_diagnostics_match_oracle(gpu_lib):
+    print("case diagnostics 64x64x512", flush=True)
+    inp, A0 = _case(64, 512)
+    o = O.oracle()
+    dims = O.dims_arr(A0.shape)
+    temp, spec = np.zeros(512), np.zeros(512)
+    bp1, bp2 = np.zeros(64), np.zeros(64)
+    o.oracle_temp_spec(O.vp(A0), O.ip(dims), O.dp(temp), O.dp(spec))
+    o.oracle_beam_profiles(O.vp(A0), O.ip(dims), O.dp(bp1), O.dp(bp2))
+    with Propagator(inp, "f64") as p:
+        p.upload(A0)
+        d = p.diagnostics()
+    np.testing.assert_allclose(d["temp"], temp, rtol=1e-12)
+    np.testing.assert_allclose(d["spec"], spec, rtol=1e-12, atol=1e-12 * spec.max())
+    np.testing.assert_allclose(d["BP1"], bp1, rtol=1e-12)
+    np.testing.assert_allclose(d["BP2"], bp2, rtol=1e-12)
+    assert abs(d["energy_sum"] - temp.sum()) < 1e-12 * temp.sum()
+
+
+def test_calibrate_matches_host(gpu_lib):
+    print("case calibrate", flush=True)
+    inp = make_inputs(configs.small(32, 512))
+    with Propagator(inp, "f64") as p:
+        p.upload(inp.A)
+        p.calibrate()
+        got = p.download()
+    assert O.rel_l2(got, inp.calibrated()) < 1e-15
+
+
+@pytest.mark.parametrize("mirror_R", [0.0, 0.3])
+def test_rho_peak_and_observables(gpu_lib, mirror_R):
+    print(f"case observables mirror_R={mirror_R}", flush=True)
+    inp, A0 = _case(32, 1024, mirror_R=mirror_R, steps_per_pass=2, passes=3, energy=3e-3)
+    ref, rho_ref = O.run_fixed(inp, A0, 0, 5)
+    with Propagator(inp, "f64") as p:
+        p.upload(A0)
+        p.run(0, 5)
+        got = p.download()
+        d = p.diagnostics()
+    assert O.rel_l2(got, ref) < TOL64
+    print(f"  rho gpu={d['rho_peak']:.6e} oracle={rho_ref:.6e}", flush=True)
+    assert rho_ref > 0
+    assert abs(d["rho_peak"] - rho_ref) <= 1e-9 * rho_ref
+
+
+def test_adaptive_matches_oracle(gpu_lib):
+    print("case adaptive", flush=True)
+    inp, A0 = _case(32, 512, mirror_R=0.0)
+    inp.params.def_err = 1e-4
+    inp.params.dzmin = 1e-7
+    o = O.oracle()
+    po = O.params_from(inp)
+    ref = A0.copy()
+    z, dz, sm = C.c_double(0.0), C.c_double(inp.params.dzz), C.c_int(1)
+    errs_o, dzs_o = np.zeros(6), np.zeros(6)
+    n_o = o.oracle_run_adaptive(O.vp(ref), C.byref(po), C.byref(z), C.byref(dz), C.byref(sm), 6, O.dp(errs_o), O.dp(dzs_o))
+    with Propagator(inp, "f64") as p:
+        p.upload(A0)
+        r = p.run_adaptive(0.0, inp.params.dzz, 1, 6)
+        got = p.download()
+    print(f"  accepted {r['accepted']}/{n_o} z={r['z']:.6g}/{z.value:.6g} err0={r['errs'][:1]} {errs_o[:1]}", flush=True)
+    assert r["accepted"] == n_o
+    np.testing.assert_allclose(r["dzs"], dzs_o[:n_o], rtol=1e-12)
+    np.testing.assert_allclose(r["errs"], errs_o[:n_o], rtol=1e-6)
+    assert abs(r["z"] - z.value) < 1e-15
+    assert O.rel_l2(got, ref) < TOL64
+
+
+def test_cfg1_full_pass_fp64(gpu_lib):
+    """64x64x512, one pass of 200 steps: field, energy, spectral width, peak rho."""
+    print("case cfg1 200 steps fp64", flush=True)
+    inp = make_inputs(configs.cfg1_cpu_64())
+    A0 = inp.calibrated()
+    ref, rho_ref = O.run_fixed(inp, A0, 0, 200)
+    with Propagator(inp, "f64") as p:
+        p.upload(inp.A)
+        p.calibrate()
+        p.run(0, 200)
+        got = p.download()
+        d = p.diagnostics()
+    dims = O.dims_arr(ref.shape)
+    temp, spec = np.zeros(512), np.zeros(512)
+    O.oracle().oracle_temp_spec(O.vp(ref), O.ip(dims), O.dp(temp), O.dp(spec))
+    err = O.rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e} rho={d['rho_peak']:.6e}/{rho_ref:.6e}", flush=True)
+    assert err <= TOL64
+    assert abs(d["energy_sum"] - temp.sum()) <= 1e-10 * temp.sum()
+    assert abs(d["rho_peak"] - rho_ref) <= 1e-9 * rho_ref
+
+
+def test_cfg3_fp32_precision_form(gpu_lib):
+    """FP32 form of the 64x64x512 workload against the FP64 oracle: per-output errors."""
+    print("case cfg3 fp32 200 steps", flush=True)
+    inp = make_inputs(configs.cfg3_precision_64())
+    A0 = inp.calibrated()
+    ref, rho_ref = O.run_fixed(inp, A0, 0, 200)
+    with Propagator(inp, "f32") as p:
+        p.upload(inp.A)
+        p.calibrate()
+        p.run(0, 200)
+        got = p.download()
+        d = p.diagnostics()
+    dims = O.dims_arr(ref.shape)
+    temp, spec = np.zeros(512), np.zeros(512)
+    O.oracle().oracle_temp_spec(O.vp(ref), O.ip(dims), O.dp(temp), O.dp(spec))
+    err = O.rel_l2(got, ref)
+    e_err = abs(d["energy_sum"] - temp.sum()) / temp.sum()
+    print(f"  field rel_l2={err:.3e} energy={e_err:.3e} rho={d['rho_peak']:.6e}/{rho_ref:.6e}", flush=True)
+    assert err <= TOL32
+    assert e_err <= 1e-5
+    assert abs(d["rho_peak"] - rho_ref) <= 1e-3 * rho_ref
