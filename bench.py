@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: split-step steps/s of the 256x256x2048 argon workload on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+One "step" is one z-step L(dz/2) N(dz) L(dz/2) of the whole 256x256x2048 field
+(Kerr + self-steepening + ADK/avalanche plasma + diffraction + dispersion),
+run through libmpc_hip.so with the field resident in HBM.  `value` is the FP32
+path (FP64 kept for the propagator phase, the ADK exponent and the density
+scan); the plain FP64 path is timed on the same inputs, and the FP32 field is
+compared with the FP64 one after the same steps (rel L2).
+
+Prints ONE JSON line.  The roofline object is for the dominant sweep kernel,
+timed with HIP events on the library's stream during a separate pass over the
+same steps; cpu_baseline is the FP64 CPU oracle on a bounded sample of the
+same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "split-step steps/s at 256²×2048 (FP32/batched vs plain FP64, ≤1e-4 rel L2)"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["t-line (nonlinear + t-FFTs)", "x-line (FFT x D x IFFT)", "y-line (FFT x D x IFFT)"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(inp, prec, warmup, steps, with_kernel_timing=True):
+    from mpc_amd import Propagator
+    p = Propagator(inp, prec)
+    p.upload(inp.A)
+    p.calibrate()
+    t0 = time.perf_counter()
+    p.run(0, warmup)
+    log(f"[bench] {prec}: warmup {warmup} steps {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    p.run(warmup, steps)                     # returns after the library stream drained
+    dt = time.perf_counter() - t0
+    log(f"[bench] {prec}: timed {steps} steps {dt:.3f}s ({steps / dt:.1f} steps/s)")
+    A = p.download()
+    kms = None
+    if with_kernel_timing:
+        p.upload(inp.A)
+        p.calibrate()
+        p.set_timing(True)
+        p.run(0, steps)
+        kms = [p.kernel_ms(i) for i in range(3)]
+        p.set_timing(False)
+        log(f"[bench] {prec}: kernel ms t/x/y = {kms}")
+    bytes_field = p.inp.A.size * (8 if prec == "f32" else 16)
+    p.close()
+    return steps / dt, dt, A, kms, bytes_field
+
+
+def cpu_baseline(inp, max_seconds=30.0):
+    """FP64 CPU oracle (test infrastructure) on the same workload: 1 warm-up
+    step, then steps until ~10 s have elapsed (at most 3)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    O.oracle()
+    A = inp.calibrated()
+    threads = O.oracle().oracle_get_threads()
+    t0 = time.perf_counter()
+    A, _ = O.run_fixed(inp, A, 0, 1)
+    log(f"[bench] cpu oracle warm-up step {time.perf_counter() - t0:.1f}s on {threads} threads")
+    n, t0 = 0, time.perf_counter()
+    while n < 3:
+        A, _ = O.run_fixed(inp, A, 1 + n, 1)
+        n += 1
+        if time.perf_counter() - t0 > min(10.0, max_seconds):
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} z-step(s) of the full 256x256x2048 workload after 1 warm-up step, "
+                      f"FP64 oracle (oracle/, OpenMP {threads} threads), {dt:.1f} s"}
+
+
+def load_traffic(prec):
+    path = os.path.join(ROOT, "profiles", f"traffic_tline_256x256x2048_{prec}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
+    ap.add_argument("--nx", type=int, default=256, help=argparse.SUPPRESS)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is designed, not built: pulses shard with no exchange).", file=sys.stderr)
+        return 2
+
+    from mpc_amd import configs, lib, make_inputs
+    if lib().mpc_device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    wl = configs.cfg2_production_256()
+    wl.nx, wl.nt = a.nx, a.nt
+    t0 = time.perf_counter()
+    inp = make_inputs(wl)
+    log(f"[bench] inputs {wl.nx}x{wl.nx}x{wl.nt} generated in {time.perf_counter() - t0:.1f}s")
+
+    sps64, dt64, A64, kms64, _ = timed_run(inp, "f64", a.warmup, a.steps)
+    sps32, dt32, A32, kms32, bf32 = timed_run(inp, "f32", a.warmup, a.steps)
+    rel = float(np.linalg.norm((A32 - A64).ravel()) / np.linalg.norm(A64.ravel()))
+    log(f"[bench] fp32 vs fp64 rel L2 after {a.warmup + a.steps} steps: {rel:.3e}")
+    del A32, A64
+
+    # dominant kernel of the FP32 path: the largest share of step time
+    shares = [kms32[i] * (1 if i == 0 else 1) for i in range(3)]
+    dom = int(np.argmax(shares))
+    per_launch_bytes = 2 * bf32           # read + write of the field
+    achieved = per_launch_bytes / (kms32[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("f32"),
+            "kernel": KERNELS[dom], "kernel_ms": round(kms32[dom], 4),
+            "all_kernels_ms": {KERNELS[i]: round(kms32[i], 4) for i in range(3)},
+            "algorithmic_bytes_per_launch": per_launch_bytes}
+
+    cpu = None
+    if not a.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(inp)
+
+    out = {
+        "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * dt32 / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian pulse + 1e-6 complex noise, argon 2 bar)",
+        "config": {"workload": f"{wl.nx}x{wl.nx}x{wl.nt} split-step, 1 mJ 300 fs, Ar 2 bar, ADK + SST on, "
+                               f"dz = 1 mm (500 steps/pass), curved mirror R = 0.3 m",
+                   "precision": "fp32 complex, fp64 propagator phase / ADK exponent / density scan",
+                   "plain_fp64_steps_per_s": round(sps64, 3),
+                   "fp32_over_fp64": round(sps32 / sps64, 3),
+                   "fp32_vs_fp64_rel_l2": rel,
+                   "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -111,7 +111,7 @@ long long mpc_field_bytes(mpc_ctx *ctx);
  * Input generator (synthetic workloads): fills the param.bin content and the
  * unscaled input field for a Gaussian pulse in x, y, t with seeded complex
  * Gaussian noise (SplitMix64 + Box-Muller).  Arrays are caller-allocated:
- * betas[Nt], wr[Ny], w[Nt], r[Ny], Are[N], Aim[N].
+ * betas[Nt], wr[Ny], w[Nt], r[Ny], A[N] in the reference layout A[t][y][x].
  */
 typedef struct {
     int    nx, ny, nt;
@@ -140,7 +140,7 @@ typedef struct {
 int mpc_adk_constants(const char *gas, double *kon);
 
 int mpc_mkinput(const mpc_input_spec *s, mpc_params *p, double *betas, double *wr,
-                double *w, double *r, double *Are, double *Aim);
+                double *w, double *r, cufftDoubleComplex *A);
 int mpc_sizeof_input_spec(void);
 
 /* param.bin / field file I/O (reference MPC.c:76-77,159-168,180-203 formats) */

@@ -75,14 +75,12 @@ def make_inputs(wl: "configs.Workload") -> Inputs:
     s.seed = wl.seed
     s.gas = wl.gas.encode()
     nx, ny, nt = wl.nx, wl.nx, wl.nt
-    N = nx * ny * nt
     betas, w = np.zeros(nt), np.zeros(nt)
     wr, r = np.zeros(ny), np.zeros(ny)
-    Are, Aim = np.zeros(N), np.zeros(N)
+    A = np.empty((nt, ny, nx), dtype=np.complex128)
     p = MpcParams()
-    check(lib().mpc_mkinput(C.byref(s), C.byref(p), _dp(betas), _dp(wr), _dp(w), _dp(r), _dp(Are), _dp(Aim)),
-          "mpc_mkinput")
-    A = (Are + 1j * Aim).reshape(nt, ny, nx)
+    check(lib().mpc_mkinput(C.byref(s), C.byref(p), _dp(betas), _dp(wr), _dp(w), _dp(r),
+                            A.ctypes.data_as(C.c_void_p)), "mpc_mkinput")
     inp = Inputs(p, betas, wr, w, r, A)
     _bind_arrays(inp)
     return inp

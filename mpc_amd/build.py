@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wno-unused-result"]
+C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wno-unused-result"]
 
 HIP_SOURCES = ["kernels.hip", "api.cpp"]
 C_SOURCES = ["host.c"]
@@ -59,7 +59,7 @@ def build_library(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lm"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-fopenmp", "-lm"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -196,10 +196,6 @@ extern "C" int mpc_create(const mpc_params *p, int precision, mpc_ctx **out) {
             rc = MPC_ENOMEM;
     }
     if (!rc) rc = ctx_set_params(c, p);
-    if (!rc) {
-        for (int i = 0; i < 8; i++)
-            if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = MPC_EHIP;
-    }
     if (rc) { mpc_destroy(c); return rc; }
     *out = c;
     return 0;
@@ -213,8 +209,10 @@ extern "C" void mpc_destroy(mpc_ctx *c) {
                     c->rho_block, c->diag_part, c->diag_line, c->diag_out, c->err_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    for (int i = 0; i < 8; i++)
-        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (auto &e : c->evs) {
+        if (e.a) (void)hipEventDestroy(e.a);
+        if (e.b) (void)hipEventDestroy(e.b);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -276,17 +274,30 @@ template <class T> static const cx<T> *user_table(mpc_ctx *c, const char *tag, c
 /* sweeps                                                                    */
 /* ------------------------------------------------------------------------ */
 static void tic(mpc_ctx *c, int which) {
-    if (c->timing) (void)hipEventRecord(c->ev[2 * which], c->stream);
+    if (!c->timing) return;
+    if (c->evn == c->evs.size()) {
+        mpc_ctx::EvRec r;
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+        c->evs.push_back(r);
+    }
+    c->evs[c->evn].which = which;
+    (void)hipEventRecord(c->evs[c->evn].a, c->stream);
 }
 static void toc(mpc_ctx *c, int which) {
-    if (!c->timing) return;
-    (void)hipEventRecord(c->ev[2 * which + 1], c->stream);
-    (void)hipEventSynchronize(c->ev[2 * which + 1]);
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, c->ev[2 * which], c->ev[2 * which + 1]) == hipSuccess) {
-        c->kms[which] += ms;
-        c->kcount[which] += 1;
+    if (!c->timing || c->evn >= c->evs.size() || c->evs[c->evn].which != which) return;
+    (void)hipEventRecord(c->evs[c->evn].b, c->stream);
+    c->evn++;
+}
+/* after the stream has drained: fold recorded launch times into the averages */
+static void resolve_timing(mpc_ctx *c) {
+    for (size_t i = 0; i < c->evn; i++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->evs[i].a, c->evs[i].b) == hipSuccess) {
+            c->kms[c->evs[i].which] += ms;
+            c->kcount[c->evs[i].which] += 1;
+        }
     }
+    c->evn = 0;
 }
 
 template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror) {
@@ -393,6 +404,7 @@ template <class T> static int run_fixed(mpc_ctx *c, int step0, int nsteps) {
 extern "C" int mpc_set_timing(mpc_ctx *c, int enable) {
     if (!c) return MPC_EINVAL;
     c->timing = enable;
+    c->evn = 0;
     for (int i = 0; i < 3; i++) { c->kms[i] = 0; c->kcount[i] = 0; }
     return 0;
 }
@@ -404,6 +416,7 @@ extern "C" double mpc_kernel_ms(mpc_ctx *c, int which) {
 static int finish(mpc_ctx *c, int rc) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->timing) resolve_timing(c);
     return 0;
 }
 

@@ -25,21 +25,23 @@ int mpc_sizeof_input_spec(void) { return (int)sizeof(mpc_input_spec); }
 /* ------------------------------------------------------------------------ */
 /* synthetic input generator                                                 */
 /* ------------------------------------------------------------------------ */
-static uint64_t splitmix64(uint64_t *s) {
-    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+/* SplitMix64 in counter form: the n-th output of the generator seeded with
+ * `seed` (state advanced n+1 times), so elements can be drawn in parallel */
+static uint64_t splitmix64_at(uint64_t seed, uint64_t n) {
+    uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
-static double u01_open(uint64_t *s) { /* (0, 1] */
-    return ((double)(splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+static double u01_open(uint64_t v) { /* (0, 1] */
+    return ((double)(v >> 11) + 1.0) * (1.0 / 9007199254740992.0);
 }
-static double u01(uint64_t *s) { /* [0, 1) */
-    return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+static double u01(uint64_t v) { /* [0, 1) */
+    return (double)(v >> 11) * (1.0 / 9007199254740992.0);
 }
 
 int mpc_mkinput(const mpc_input_spec *s, mpc_params *p, double *betas, double *wr, double *w,
-                double *r, double *Are, double *Aim) {
+                double *r, cufftDoubleComplex *A) {
     if (!s || !p || s->nx <= 0 || s->ny <= 0 || s->nt <= 0) return MPC_EINVAL;
     if (s->nx != s->ny || s->steps_per_pass <= 0 || s->passes <= 0) return MPC_EINVAL;
     const int nx = s->nx, ny = s->ny, nt = s->nt;
@@ -84,27 +86,38 @@ int mpc_mkinput(const mpc_input_spec *s, mpc_params *p, double *betas, double *w
         r[j] = (j - ny / 2) * dx;
     }
     p->betas = betas; p->wr = wr; p->w = w; p->r = r;
-    /* Gaussian in x, y, t: exp(-(x^2+y^2)/waist^2) exp(-2 ln2 t^2/tau^2) */
+    /* Gaussian in x, y, t: exp(-(x^2+y^2)/waist^2) exp(-2 ln2 t^2/tau^2), plus
+     * complex Gaussian noise (Box-Muller on draws 2m, 2m+1 for element m) */
     const double sig = s->noise_rel; /* max|A| = 1 at the grid centre */
-    uint64_t st = s->seed;
-    double esum = 0.0;
-    long m = 0;
+    const uint64_t seed = s->seed;
+    double *fx = (double *)malloc(sizeof(double) * (size_t)nx);
+    double *ft = (double *)malloc(sizeof(double) * (size_t)nt);
+    for (int i = 0; i < nx; i++) fx[i] = exp(-(r[i] * r[i]) / (s->waist * s->waist));
     for (int k = 0; k < nt; k++) {
         const double t = (k - nt / 2) * dt;
-        const double ft = exp(-2.0 * log(2.0) * t * t / (s->tau_fwhm * s->tau_fwhm));
-        for (int j = 0; j < ny; j++) {
-            const double y = r[j];
-            for (int i = 0; i < nx; i++, m++) {
-                const double x = r[i];
-                const double a = exp(-(x * x + y * y) / (s->waist * s->waist)) * ft;
-                const double u1 = u01_open(&st), u2 = u01(&st);
-                const double rad = sqrt(-2.0 * log(u1));
-                Are[m] = a + sig * rad * cos(2.0 * M_PI * u2);
-                Aim[m] = sig * rad * sin(2.0 * M_PI * u2);
-                esum += Are[m] * Are[m] + Aim[m] * Aim[m];
-            }
-        }
+        ft[k] = exp(-2.0 * log(2.0) * t * t / (s->tau_fwhm * s->tau_fwhm));
     }
+    double esum = 0.0;
+    #pragma omp parallel for schedule(static) reduction(+ : esum)
+    for (int k = 0; k < nt; k++) {
+        double ek = 0.0;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const long m = ((long)k * ny + j) * nx + i;
+                const double a = fx[i] * fx[j] * ft[k];
+                const double u1 = u01_open(splitmix64_at(seed, 2 * (uint64_t)m));
+                const double u2 = u01(splitmix64_at(seed, 2 * (uint64_t)m + 1));
+                const double rad = sqrt(-2.0 * log(u1));
+                const double re = a + sig * rad * cos(2.0 * M_PI * u2);
+                const double im = sig * rad * sin(2.0 * M_PI * u2);
+                A[m].x = re;
+                A[m].y = im;
+                ek += re * re + im * im;
+            }
+        esum += ek;
+    }
+    free(fx);
+    free(ft);
     p->Esig = esum * HBAR * w0 * dx * dx * dt;
     return MPC_OK;
 }

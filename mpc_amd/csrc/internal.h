@@ -115,9 +115,12 @@ struct mpc_ctx_s {
      * multiplied by ascale^2 instead. */
     double ascale = 1.0;
     double rho_peak = 0.0;
-    /* timing */
+    /* per-launch timing: event pairs recorded on the library stream around
+     * each sweep launch, resolved once the stream has drained */
+    struct EvRec { hipEvent_t a = nullptr, b = nullptr; int which = 0; };
     int timing = 0;
-    hipEvent_t ev[8] = {};
+    std::vector<EvRec> evs;
+    size_t evn = 0;
     double kms[3] = {0, 0, 0};
     long kcount[3] = {0, 0, 0};
 };
