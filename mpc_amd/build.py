@@ -61,8 +61,11 @@ def build_library(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-fopenmp", "-lm"])
     os.replace(tmp, LIB)
+    # the run driver (MPC.c's replacement), linked against the library
+    _run(["gcc", *C_FLAGS, "-o", os.path.join(HERE, "mpc"), os.path.join(CSRC, "mpc_main.c"),
+          f"-L{HERE}", "-lmpc_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", os.path.join(HERE, "mpc"))
     return LIB
 
 

@@ -150,12 +150,18 @@ template <int DIR, class T> struct Dft<16, DIR, T> {
     static MPC_HD void run(cx<T> *x) { DftCT<4, 4, DIR, T>::run(x); }
 };
 
+/* LDS index with one padding slot every PAD elements (PAD = 0: none).  The
+ * padding breaks the power-of-two strides of the early Stockham stages so
+ * consecutive lanes land in different LDS banks. */
+template <int PAD> MPC_HD int padi(int i) { return PAD ? i + i / PAD : i; }
+
 /*
  * One Stockham stage.  Ns = product of the radices of earlier stages.
  * v: the thread's E registers (x[tid + TPL*m]); lds: the line's exchange
- * buffer, element idx at lds[idx * ls]; tw: N-entry table exp(-2 pi i k/N).
+ * buffer, element idx at lds[padi<PAD>(idx) * ls]; tw: N-entry table
+ * exp(-2 pi i k/N).
  */
-template <class T, int N, int E, int R, int Ns, int DIR, bool LAST>
+template <class T, int N, int E, int R, int Ns, int DIR, bool LAST, int PAD = 0>
 MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
     constexpr int TPL = N / E;
     constexpr int B = E / R;
@@ -182,20 +188,28 @@ MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *t
         } else {
             const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[(base + r * Ns) * ls] = u[r];
+            for (int r = 0; r < R; ++r) lds[padi<PAD>(base + r * Ns) * ls] = u[r];
         }
     }
 }
 
-template <class T, int N, int E>
+template <class T, int N, int E, int PAD = 0>
 MPC_HD void fft_gather(cx<T> (&v)[E], int tid, const cx<T> *lds, int ls) {
     constexpr int TPL = N / E;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[(tid + TPL * m) * ls];
+    for (int m = 0; m < E; ++m) v[m] = lds[padi<PAD>(tid + TPL * m) * ls];
 }
 
 /* radix plans: a compile-time list of radices whose product is N */
 template <int... Rs> struct Radices {};
+
+/* t lines: 8 elements per thread keeps the fused nonlinear kernel's register
+ * footprint low (two line copies live at once) */
+template <int N> struct TPlan;
+template <> struct TPlan<4096> { static constexpr int E = 8; using R = Radices<8, 8, 8, 8>; };
+template <> struct TPlan<2048> { static constexpr int E = 8; using R = Radices<8, 8, 8, 4>; };
+template <> struct TPlan<1024> { static constexpr int E = 8; using R = Radices<8, 8, 8, 2>; };
+template <> struct TPlan<512>  { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
 
 template <int N> struct Plan;
 /* t lines */

@@ -26,32 +26,41 @@ namespace mpc {
 /* ---------------------------------------------------------------------- */
 /* FFT driver: unrolls Plan<N>::R into stages with LDS exchange between    */
 /* ---------------------------------------------------------------------- */
-template <class T, int N, int E, int DIR, int Ns, int R, int... Rest>
+template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
 struct DevFft {
     static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
                                                const cx<T> *tw) {
         constexpr bool last = sizeof...(Rest) == 0;
-        fft_stage<T, N, E, R, Ns, DIR, last>(v, tid, lds, ls, tw);
+        fft_stage<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds, ls, tw);
         if constexpr (!last) {
             __syncthreads();
-            fft_gather<T, N, E>(v, tid, lds, ls);
+            fft_gather<T, N, E, PAD>(v, tid, lds, ls);
             __syncthreads();
-            DevFft<T, N, E, DIR, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
+            DevFft<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
         }
     }
 };
-template <class T, int N, int DIR, class Rad> struct DevFftP;
-template <class T, int N, int DIR, int... Rs> struct DevFftP<T, N, DIR, Radices<Rs...>> {
-    static __device__ __forceinline__ void run(cx<T> (&v)[Plan<N>::E], int tid, cx<T> *lds,
-                                               int ls, const cx<T> *tw) {
-        DevFft<T, N, Plan<N>::E, DIR, 1, Rs...>::run(v, tid, lds, ls, tw);
+template <class T, int N, int E, int DIR, int PAD, class Rad> struct DevFftP;
+template <class T, int N, int E, int DIR, int PAD, int... Rs> struct DevFftP<T, N, E, DIR, PAD, Radices<Rs...>> {
+    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+        DevFft<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, tw);
     }
 };
-template <class T, int N, int DIR>
-__device__ __forceinline__ void line_fft(cx<T> (&v)[Plan<N>::E], int tid, cx<T> *lds, int ls,
-                                         const cx<T> *tw) {
-    DevFftP<T, N, DIR, typename Plan<N>::R>::run(v, tid, lds, ls, tw);
+/* line FFT with plan PL (PL::E elements per thread, radix list PL::R) */
+template <class T, int N, class PL, int DIR, int PAD = 0>
+__device__ __forceinline__ void line_fft(cx<T> (&v)[PL::E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+    DevFftP<T, N, PL::E, DIR, PAD, typename PL::R>::run(v, tid, lds, ls, tw);
 }
+
+/* t-line geometry: plan, LDS padding and lines per 256-thread block */
+template <int NT> struct TGeo {
+    using PL = TPlan<NT>;
+    static constexpr int E = PL::E, TPL = NT / E;
+    static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
+    static constexpr int THREADS = LPB * TPL;
+    static constexpr int PAD = 8;                       /* LDS pad slot every 8 elements */
+    static constexpr int FFT_ELEMS = NT + NT / PAD;
+};
 
 /* ---------------------------------------------------------------------- */
 /* plasma model pieces (FP64 in both precisions)                           */
@@ -96,7 +105,9 @@ __device__ __forceinline__ PEl plasma_el(double a2, const PlasmaK &k, bool rates
     return r;
 }
 
-__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+/* plasma arrays: one pad slot per E-element segment, so the per-thread
+ * consecutive segments of the scan start in different LDS banks */
+template <int NT> __device__ __forceinline__ int padp(int i) { return i + i / TGeo<NT>::E; }
 
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
@@ -107,14 +118,14 @@ __device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
  */
 template <int NT>
 __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const double *C, double *scr) {
-    constexpr int E = Plan<NT>::E, TPL = NT / E;
+    constexpr int E = TGeo<NT>::E, TPL = NT / E;
     constexpr int W = TPL < 64 ? TPL : 64;
     constexpr int NW = (TPL + 63) / 64;
     const int lo = tid * E;
     double gm = 1.0, gc = 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = pad16(lo + i);
+        const int k = padp<NT>(lo + i);
         const double mk = M[k], ck = C[k];
         gc = mk * gc + ck;
         gm = mk * gm;
@@ -144,7 +155,7 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const dou
     double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = pad16(lo + i);
+        const int k = padp<NT>(lo + i);
         R = M[k] * R + C[k];
         M[k] = R;
     }
@@ -152,8 +163,8 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const dou
 }
 
 template <int NT> struct PlasmaLds {
-    static constexpr int E = Plan<NT>::E, TPL = NT / E, NW = (TPL + 63) / 64;
-    static constexpr int ARR = NT + NT / 16;           /* padded doubles per array */
+    static constexpr int E = TGeo<NT>::E, TPL = NT / E, NW = (TPL + 63) / 64;
+    static constexpr int ARR = NT + NT / E;            /* padded doubles per array */
     static constexpr int SCR = 2 * NW + TPL;
     static constexpr int BYTES = (2 * ARR + SCR) * 8;
 };
@@ -168,7 +179,7 @@ template <int NT> struct PlasmaLds {
  * sink(m, F, tau) receives the result per element.
  */
 template <class T, int NT, class Sink>
-__device__ __forceinline__ void plasma_line(const cx<T> (&A)[Plan<NT>::E], int tid, double *lds,
+__device__ __forceinline__ void plasma_line(const cx<T> (&A)[TGeo<NT>::E], int tid, double *lds,
                                             const PlasmaK &pk, double &rho_max, Sink sink) {
     using L = PlasmaLds<NT>;
     constexpr int E = L::E, TPL = L::TPL;
@@ -180,8 +191,8 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[Plan<NT>::E], int t
         const int k = tid + TPL * m;
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
         const PEl e = plasma_el(a2, pk, true);
-        Lm[pad16(k)] = 1.0 - e.a;
-        Lc[pad16(k)] = e.b * (1.0 - e.a) + e.a;
+        Lm[padp<NT>(k)] = 1.0 - e.a;
+        Lc[padp<NT>(k)] = e.b * (1.0 - e.a) + e.a;
         cc[m] = e.c;
         anyc |= (e.c != 0.0);
     }
@@ -190,25 +201,25 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[Plan<NT>::E], int t
     const int block_anyc = __syncthreads_or(anyc);
     double u[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) u[m] = -expm1(-Lm[pad16(tid + TPL * m)]);
+    for (int m = 0; m < E; ++m) u[m] = -expm1(-Lm[padp<NT>(tid + TPL * m)]);
     if (block_anyc) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int k = tid + TPL * m;
-            const double un = (k + 1 < NT) ? -expm1(-Lm[pad16(k + 1)]) : 0.0;
-            Lc[pad16(k)] = 1.0 - cc[m];   /* M' */
+            const double un = (k + 1 < NT) ? -expm1(-Lm[padp<NT>(k + 1)]) : 0.0;
+            Lc[padp<NT>(k)] = 1.0 - cc[m];   /* M' */
             cc[m] = cc[m] * un;           /* C' */
         }
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) Lm[pad16(tid + TPL * m)] = cc[m];
+        for (int m = 0; m < E; ++m) Lm[padp<NT>(tid + TPL * m)] = cc[m];
         __syncthreads();
         affine_suffix_scan<NT>(tid, Lc, Lm, scr);     /* Lc <- ions2 */
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
-        const double i2 = block_anyc ? Lc[pad16(k)] : 0.0;
+        const double i2 = block_anyc ? Lc[padp<NT>(k)] : 0.0;
         const double i1 = u[m] - i2;
         const double fr = i1 + 2.0 * i2;
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
@@ -223,17 +234,79 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[Plan<NT>::E], int t
 /* t-line kernel                                                           */
 /* ---------------------------------------------------------------------- */
 template <class T, int NT, bool PLASMA> struct TCfg {
-    static constexpr int E = Plan<NT>::E, TPL = NT / E;
-    static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
-    static constexpr int THREADS = LPB * TPL;
-    static constexpr int FFT_BYTES = NT * (int)sizeof(cx<T>);
+    using G = TGeo<NT>;
+    static constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB, THREADS = G::THREADS;
+    static constexpr int FFT_BYTES = G::FFT_ELEMS * (int)sizeof(cx<T>);
     static constexpr int PL_BYTES = PLASMA ? PlasmaLds<NT>::BYTES : 0;
     static constexpr int LINE_BYTES = ((FFT_BYTES > PL_BYTES ? FFT_BYTES : PL_BYTES) + 15) / 16 * 16;
 };
 
+/*
+ * sin/cos of the per-step nonlinear phase (small in practice).  Cody-Waite
+ * reduction by pi/2 with a three-part constant, then Taylor polynomials on
+ * [-pi/4, pi/4] (truncation below 1e-19 in FP64, 2e-9 in FP32) and the
+ * quadrant swap.  |x| > 1e5 takes the library routine instead.
+ */
+__device__ __noinline__ void sincos_big(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __noinline__ void sincos_bigf(float x, float *s, float *c) { sincosf(x, s, c); }
+
 template <class T> __device__ __forceinline__ void sincos_t(T x, T *s, T *c);
-template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) { sincos(x, s, c); }
-template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) {
+    if (!(fabs(x) <= 1e5)) { sincos_big(x, s, c); return; }
+    const double k = rint(x * 0.63661977236758134308);          /* 2/pi */
+    double r = fma(-k, 1.57079632673412561417e+00, x);            /* pi/2 = h + m + l */
+    r = fma(-k, 6.07710050630396597660e-11, r);
+    r = fma(-k, 2.02226624879595063154e-21, r);
+    const double r2 = r * r;
+    /* sin r = r - r^3/3! + ... ; cos r = 1 - r^2/2! + ... (Horner in r^2) */
+    double sp = -1.0 / 121645100408832000.0;                      /* -1/19! */
+    sp = fma(sp, r2, 1.0 / 355687428096000.0);                    /* 1/17! */
+    sp = fma(sp, r2, -1.0 / 1307674368000.0);                     /* -1/15! */
+    sp = fma(sp, r2, 1.0 / 6227020800.0);                         /* 1/13! */
+    sp = fma(sp, r2, -1.0 / 39916800.0);                          /* -1/11! */
+    sp = fma(sp, r2, 1.0 / 362880.0);                             /* 1/9! */
+    sp = fma(sp, r2, -1.0 / 5040.0);                              /* -1/7! */
+    sp = fma(sp, r2, 1.0 / 120.0);                                /* 1/5! */
+    sp = fma(sp, r2, -1.0 / 6.0);                                 /* -1/3! */
+    const double sr = fma(sp * r2, r, r);
+    double cp = 1.0 / 6402373705728000.0;                         /* 1/18! */
+    cp = fma(cp, r2, -1.0 / 20922789888000.0);                    /* -1/16! */
+    cp = fma(cp, r2, 1.0 / 87178291200.0);                        /* 1/14! */
+    cp = fma(cp, r2, -1.0 / 479001600.0);                         /* -1/12! */
+    cp = fma(cp, r2, 1.0 / 3628800.0);                            /* 1/10! */
+    cp = fma(cp, r2, -1.0 / 40320.0);                             /* -1/8! */
+    cp = fma(cp, r2, 1.0 / 720.0);                                /* 1/6! */
+    cp = fma(cp, r2, -1.0 / 24.0);                                /* -1/4! */
+    cp = fma(cp, r2, 0.5);
+    const double cr = fma(-cp, r2, 1.0);
+    const int q = ((int)k) & 3;
+    const double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) {
+    if (!(fabsf(x) <= 1e5f)) { sincos_bigf(x, s, c); return; }
+    const float k = rintf(x * 0.636619772f);
+    float r = fmaf(-k, 1.5703125f, x);                            /* pi/2 in three parts */
+    r = fmaf(-k, 4.8351287841796875e-4f, r);
+    r = fmaf(-k, 3.13916473e-7f, r);
+    const float r2 = r * r;
+    float sp = 1.0f / 362880.0f;                                  /* Taylor to r^9 */
+    sp = fmaf(sp, r2, -1.0f / 5040.0f);
+    sp = fmaf(sp, r2, 1.0f / 120.0f);
+    sp = fmaf(sp, r2, -1.0f / 6.0f);
+    const float sr = fmaf(sp * r2, r, r);
+    float cp = -1.0f / 3628800.0f;                                /* Taylor to r^10 */
+    cp = fmaf(cp, r2, 1.0f / 40320.0f);
+    cp = fmaf(cp, r2, -1.0f / 720.0f);
+    cp = fmaf(cp, r2, 1.0f / 24.0f);
+    cp = fmaf(cp, r2, -0.5f);
+    const float cr = fmaf(cp, r2, 1.0f);
+    const int q = ((int)k) & 3;
+    const float ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
 template <class T> __device__ __forceinline__ T exp_t(T x);
 template <> __device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
@@ -252,7 +325,7 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
     cx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = g[tid + TPL * m];
-    if (!a.in_freq) line_fft<T, NT, -1>(v, tid, L, 1, a.tw);
+    if (!a.in_freq) line_fft<T, NT, TPlan<NT>, -1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
 
@@ -264,9 +337,9 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
                 const T wk = a.wfft[tid + TPL * m];
                 d[m] = mk(wk * v[m].im, -wk * v[m].re);   /* -i omega A(omega) */
             }
-            line_fft<T, NT, +1>(d, tid, L, 1, a.tw);      /* dA */
+            line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(d, tid, L, 1, a.tw);   /* dA */
         }
-        line_fft<T, NT, +1>(v, tid, L, 1, a.tw);          /* A  */
+        line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);       /* A  */
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
@@ -314,13 +387,13 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
             const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
             v[m] = mk(er * z.re, er * z.im);
         }
-        line_fft<T, NT, -1>(v, tid, L, 1, a.tw);
+        line_fft<T, NT, TPlan<NT>, -1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
     }
     if (a.Q) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
     }
-    if (a.out_time) line_fft<T, NT, +1>(v, tid, L, 1, a.tw);
+    if (a.out_time) line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
 }
@@ -352,10 +425,10 @@ template int launch_tline<float>(const TArgs<float> &, int, bool, hipStream_t);
 int tline_lines_per_block(int nt, bool plasma, int prec) {
     (void)plasma; (void)prec;
     switch (nt) {
-    case 512: return TCfg<double, 512, true>::LPB;
-    case 1024: return TCfg<double, 1024, true>::LPB;
-    case 2048: return TCfg<double, 2048, true>::LPB;
-    case 4096: return TCfg<double, 4096, true>::LPB;
+    case 512: return TGeo<512>::LPB;
+    case 1024: return TGeo<1024>::LPB;
+    case 2048: return TGeo<2048>::LPB;
+    case 4096: return TGeo<4096>::LPB;
     default: return 1;
     }
 }
@@ -383,10 +456,10 @@ __global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
     cx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = base[(long)(tid + TPL * m) * a.s_line];
-    line_fft<T, N, -1>(v, tid, lds + gcol, G, a.tw);
+    line_fft<T, N, Plan<N>, -1>(v, tid, lds + gcol, G, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.D[tid + TPL * m]);
-    line_fft<T, N, +1>(v, tid, lds + gcol, G, a.tw);
+    line_fft<T, N, Plan<N>, +1>(v, tid, lds + gcol, G, a.tw);
     if (a.M) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
@@ -395,7 +468,7 @@ __global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
     for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
 }
 
-bool axis_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
 
 template <class T, int N> static int axis_go(const AArgs<T> &a, int outer, hipStream_t s) {
     using C = ACfg<T, N>;
@@ -405,6 +478,8 @@ template <class T, int N> static int axis_go(const AArgs<T> &a, int outer, hipSt
 }
 template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, hipStream_t s) {
     switch (n) {
+    case 16:  return axis_go<T, 16>(a, outer, s);
+    case 32:  return axis_go<T, 32>(a, outer, s);
     case 64:  return axis_go<T, 64>(a, outer, s);
     case 128: return axis_go<T, 128>(a, outer, s);
     case 192: return axis_go<T, 192>(a, outer, s);
@@ -426,14 +501,18 @@ __global__ void __launch_bounds__(256) k_to_internal(const double *__restrict__ 
     const int x0 = blockIdx.x * 32, t0 = blockIdx.y * 32, y = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   /* 32 x 8 */
     for (int i = ty; i < 32; i += 8) {
-        const long s = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
-        tile[i][tx] = reinterpret_cast<const double2 *>(src)[s];
+        if (x0 + tx < nx) {
+            const long s = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
+            tile[i][tx] = reinterpret_cast<const double2 *>(src)[s];
+        }
     }
     __syncthreads();
     for (int i = ty; i < 32; i += 8) {
-        const double2 e = tile[tx][i];                           /* t = t0+tx, x = x0+i */
-        const long d = ((long)y * nx + x0 + i) * nt + t0 + tx;
-        dst[d] = mk((T)(e.x * scale), (T)(e.y * scale));
+        if (x0 + i < nx) {
+            const double2 e = tile[tx][i];                       /* t = t0+tx, x = x0+i */
+            const long d = ((long)y * nx + x0 + i) * nt + t0 + tx;
+            dst[d] = mk((T)(e.x * scale), (T)(e.y * scale));
+        }
     }
 }
 template <class T>
@@ -443,23 +522,27 @@ __global__ void __launch_bounds__(256) k_to_reference(const cx<T> *__restrict__ 
     const int x0 = blockIdx.x * 32, t0 = blockIdx.y * 32, y = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int i = ty; i < 32; i += 8) {
-        const cx<T> e = src[((long)y * nx + x0 + i) * nt + t0 + tx];   /* x = x0+i, t = t0+tx */
-        tile[tx][i] = make_double2((double)e.re * scale, (double)e.im * scale);
+        if (x0 + i < nx) {
+            const cx<T> e = src[((long)y * nx + x0 + i) * nt + t0 + tx];   /* x = x0+i, t = t0+tx */
+            tile[tx][i] = make_double2((double)e.re * scale, (double)e.im * scale);
+        }
     }
     __syncthreads();
     for (int i = ty; i < 32; i += 8) {
-        const long d = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
-        reinterpret_cast<double2 *>(dst)[d] = tile[i][tx];
+        if (x0 + tx < nx) {
+            const long d = ((long)(t0 + i) * ny + y) * nx + x0 + tx;
+            reinterpret_cast<double2 *>(dst)[d] = tile[i][tx];
+        }
     }
 }
 template <class T> int launch_to_internal(const double *src, cx<T> *dst, int nx, int ny, int nt, double scale, hipStream_t s) {
-    if (nx % 32 || nt % 32) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_to_internal<T>), dim3(nx / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
+    if (nt % 32) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_to_internal<T>), dim3((nx + 31) / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T> int launch_to_reference(const cx<T> *src, double *dst, int nx, int ny, int nt, double scale, hipStream_t s) {
-    if (nx % 32 || nt % 32) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_to_reference<T>), dim3(nx / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
+    if (nt % 32) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_to_reference<T>), dim3((nx + 31) / 32, nt / 32, ny), dim3(256), 0, s, src, dst, nx, ny, nt, scale);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template int launch_to_internal<double>(const double *, cx<double> *, int, int, int, double, hipStream_t);
@@ -502,8 +585,9 @@ template int launch_scale<float>(cx<float> *, long, float, hipStream_t);
 template <class T, int NT>
 __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> *tw, long nlines,
                                                     double *line_power, double *temp_part, double *spec_part) {
-    constexpr int E = Plan<NT>::E, TPL = NT / E, LPB = (256 / TPL) > 0 ? 256 / TPL : 1;
-    __shared__ __align__(16) cx<T> lds[LPB * NT];
+    using G = TGeo<NT>;
+    constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB;
+    __shared__ __align__(16) cx<T> lds[LPB * G::FFT_ELEMS];
     __shared__ double accT[NT], accS[NT];
     __shared__ double lp[LPB][TPL / 64 + 1];
     const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
@@ -526,16 +610,17 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
             pw += p;
         }
         /* line power: reduce over the TPL threads of the line */
-        for (int o = (TPL < 64 ? TPL : 64) / 2; o > 0; o >>= 1) pw += __shfl_down(pw, o, TPL < 64 ? TPL : 64);
+        constexpr int W = TPL < 64 ? TPL : 64;
+        for (int o = W / 2; o > 0; o >>= 1) pw += __shfl_down(pw, o, W);
         if (tid % 64 == 0) lp[lslot][tid / 64] = pw;
-        line_fft<T, NT, -1>(v, tid, lds + lslot * NT, 1, tw);
+        line_fft<T, NT, TPlan<NT>, -1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
         __syncthreads();
         if (tid == 0) {
-            double s = 0.0;
-            for (int i = 0; i < (TPL + 63) / 64; ++i) s += lp[lslot][i];
-            line_power[line] = s;
+            double sum = 0.0;
+            for (int i = 0; i < (TPL + 63) / 64; ++i) sum += lp[lslot][i];
+            line_power[line] = sum;
         }
         __syncthreads();
     }
@@ -554,9 +639,9 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
 }
 template <class T, int NT>
 static int diag_go(const cx<T> *f, const cx<T> *tw, long nlines, double *lp, double *tp, double *sp, int nb, hipStream_t s) {
-    constexpr int TPL = NT / Plan<NT>::E, LPB = (256 / TPL) > 0 ? 256 / TPL : 1;
-    if (nlines % LPB) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_diag_lines<T, NT>), dim3(nb), dim3(LPB * TPL), 0, s, f, tw, nlines, lp, tp, sp);
+    using G = TGeo<NT>;
+    if (nlines % G::LPB) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_diag_lines<T, NT>), dim3(nb), dim3(G::THREADS), 0, s, f, tw, nlines, lp, tp, sp);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines, double *lp,

@@ -10,34 +10,34 @@
 
 using namespace mpc;
 
-template <class T, int N, int E, int DIR, int Ns, int R, int... Rest>
+template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
 struct HostRun {
     static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const cx<T> *tw) {
         constexpr int TPL = N / E;
         constexpr bool last = sizeof...(Rest) == 0;
         for (int tid = 0; tid < TPL; ++tid) {
             cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
-            fft_stage<T, N, E, R, Ns, DIR, last>(v, tid, lds.data(), 1, tw);
+            fft_stage<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds.data(), 1, tw);
         }
         if constexpr (!last) {
             for (int tid = 0; tid < TPL; ++tid) {
                 cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
-                fft_gather<T, N, E>(v, tid, lds.data(), 1);
+                fft_gather<T, N, E, PAD>(v, tid, lds.data(), 1);
             }
-            HostRun<T, N, E, DIR, Ns * R, Rest...>::run(V, lds, tw);
+            HostRun<T, N, E, DIR, PAD, Ns * R, Rest...>::run(V, lds, tw);
         }
     }
 };
-template <class T, int N, int DIR, class Rad> struct Unpack;
-template <class T, int N, int DIR, int... Rs> struct Unpack<T, N, DIR, Radices<Rs...>> {
+template <class T, int N, int E, int DIR, int PAD, class Rad> struct Unpack;
+template <class T, int N, int E, int DIR, int PAD, int... Rs> struct Unpack<T, N, E, DIR, PAD, Radices<Rs...>> {
     static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const cx<T> *tw) {
-        HostRun<T, N, Plan<N>::E, DIR, 1, Rs...>::run(V, lds, tw);
+        HostRun<T, N, E, DIR, PAD, 1, Rs...>::run(V, lds, tw);
     }
 };
 
-template <class T, int N, int DIR> double check(unsigned seed) {
-    constexpr int E = Plan<N>::E, TPL = N / E;
-    std::vector<cx<T>> tw(N), lds(N), V((size_t)TPL * E);
+template <class T, int N, int DIR, class PL = Plan<N>, int PAD = 0> double check(unsigned seed) {
+    constexpr int E = PL::E, TPL = N / E;
+    std::vector<cx<T>> tw(N), lds(N + (PAD ? N / PAD : 0)), V((size_t)TPL * E);
     for (int k = 0; k < N; ++k) {
         long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
         tw[k] = mk((T)cosl(a), (T)sinl(a));
@@ -47,7 +47,7 @@ template <class T, int N, int DIR> double check(unsigned seed) {
     for (int i = 0; i < N; ++i) { xr[i] = (T)(rand() / (double)RAND_MAX - 0.5); xi[i] = (T)(rand() / (double)RAND_MAX - 0.5); }
     for (int tid = 0; tid < TPL; ++tid)
         for (int m = 0; m < E; ++m) V[(size_t)tid * E + m] = mk((T)xr[tid + TPL * m], (T)xi[tid + TPL * m]);
-    Unpack<T, N, DIR, typename Plan<N>::R>::run(V, lds, tw.data());
+    Unpack<T, N, E, DIR, PAD, typename PL::R>::run(V, lds, tw.data());
     double num = 0, den = 0;
     for (int k = 0; k < N; ++k) {
         long double sr = 0, si = 0;
@@ -73,8 +73,17 @@ template <int N> int run_n() {
     return bad;
 }
 
+template <int N> int run_t() {
+    /* t-line plans with the padded LDS layout used by the fused kernel */
+    double e1 = check<double, N, -1, TPlan<N>, 8>(N), e2 = check<double, N, +1, TPlan<N>, 8>(N + 1);
+    double e3 = check<float, N, -1, TPlan<N>, 8>(N + 2), e4 = check<float, N, +1, TPlan<N>, 8>(N + 3);
+    printf("t-plan N=%5d f64 fwd %.3e inv %.3e  f32 fwd %.3e inv %.3e\n", N, e1, e2, e3, e4);
+    return (e1 > 1e-14 || e2 > 1e-14 || e3 > 2e-6 || e4 > 2e-6) ? 1 : 0;
+}
+
 int main() {
     int bad = 0;
+    bad |= run_t<512>(); bad |= run_t<1024>(); bad |= run_t<2048>(); bad |= run_t<4096>();
     bad |= run_n<16>(); bad |= run_n<32>(); bad |= run_n<64>(); bad |= run_n<128>();
     bad |= run_n<192>(); bad |= run_n<256>(); bad |= run_n<512>(); bad |= run_n<1024>();
     bad |= run_n<2048>(); bad |= run_n<4096>();

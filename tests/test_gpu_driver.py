@@ -1,0 +1,69 @@
+"""GPU test of the run driver (mpc_amd/mpc, the MPC.c replacement): files,
+dump formats and kill-after-checkpoint / resume (-r) against an
+uninterrupted run, which must agree bit for bit (same device arithmetic)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mpc_amd import configs, make_inputs
+from mpc_amd.mkinput import write_inputs
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MPC = os.path.join(ROOT, "mpc_amd", "mpc")
+
+
+def _run(args, cwd):
+    print("driver:", " ".join(args), flush=True)
+    p = subprocess.run([MPC, *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return p.stdout
+
+
+def test_driver_fixed_checkpoint_resume(gpu_lib, tmp_path):
+    wl = configs.small(16, 512, steps_per_pass=10, passes=2, mirror_R=0.3)
+    inp = make_inputs(wl)
+    base = str(tmp_path / "data")
+    rundir = write_inputs(inp, base, 0)
+    alt = str(tmp_path / "alt")
+    shutil.copytree(tmp_path, tmp_path.parent / (tmp_path.name + "_copy"))
+    copy = str(tmp_path.parent / (tmp_path.name + "_copy") / "data")
+    args = ["-d", base, "-n", "0", "-F", "10", "-R", "0.3", "-k", "1"]
+    _run(args, str(tmp_path))
+    full = np.fromfile(os.path.join(rundir, "out.bin"))
+    # interrupted run: stop after 15 dumps (last checkpoint at step 10), resume
+    argsc = ["-d", copy, "-n", "0", "-F", "10", "-R", "0.3", "-k", "1"]
+    _run(argsc + ["-m", "15"], os.path.dirname(copy))
+    _run(argsc + ["-r"], os.path.dirname(copy))
+    crun = copy + "0000"
+    resumed = np.fromfile(os.path.join(crun, "out.bin"))
+    assert np.array_equal(full, resumed)
+    for f in ("bp1.bin", "bp2.bin", "temp.bin", "spec.bin", "other.bin"):
+        a = open(os.path.join(rundir, f), "rb").read()
+        b = open(os.path.join(crun, f), "rb").read()
+        assert a == b, f
+    # formats: 20 dumps of Nx / Nt doubles; out.bin = 2N doubles
+    assert os.path.getsize(os.path.join(rundir, "bp1.bin")) == 20 * 16 * 8
+    assert os.path.getsize(os.path.join(rundir, "temp.bin")) == 20 * 512 * 8
+    assert full.size == 2 * 16 * 16 * 512
+    assert os.path.getsize(os.path.join(rundir, "resparam.bin")) == 32
+    # and the field agrees with the oracle's fixed-step run
+    ref, _ = O.run_fixed(inp, inp.calibrated(), 0, 20)
+    got = full.view(np.complex128).reshape(ref.shape)
+    assert O.rel_l2(got, ref) < 1e-10
+
+
+def test_driver_adaptive_runs(gpu_lib, tmp_path):
+    wl = configs.small(16, 512, steps_per_pass=10, passes=1, mirror_R=0.0)
+    wl.def_err = 1e-4
+    inp = make_inputs(wl)
+    base = str(tmp_path / "data")
+    rundir = write_inputs(inp, base, 0)
+    out = _run(["-d", base, "-n", "0", "-m", "4"], str(tmp_path))
+    assert "local error" in out or "We are here" in out
+    lines = open(os.path.join(rundir, "other.bin")).read().strip().splitlines()
+    assert len(lines) == 4 and all("\t" in l for l in lines)

@@ -95,7 +95,7 @@ def test_calibrate_matches_host(gpu_lib):
 @pytest.mark.parametrize("mirror_R", [0.0, 0.3])
 def test_rho_peak_and_observables(gpu_lib, mirror_R):
     print(f"case observables mirror_R={mirror_R}", flush=True)
-    inp, A0 = _case(32, 1024, mirror_R=mirror_R, steps_per_pass=2, passes=3, energy=3e-3)
+    inp, A0 = _case(32, 1024, mirror_R=mirror_R, steps_per_pass=2, passes=3, energy=1e-3)
     ref, rho_ref = O.run_fixed(inp, A0, 0, 5)
     with Propagator(inp, "f64") as p:
         p.upload(A0)
@@ -172,5 +172,5 @@ def test_cfg3_fp32_precision_form(gpu_lib):
     e_err = abs(d["energy_sum"] - temp.sum()) / temp.sum()
     print(f"  field rel_l2={err:.3e} energy={e_err:.3e} rho={d['rho_peak']:.6e}/{rho_ref:.6e}", flush=True)
     assert err <= TOL32
-    assert e_err <= 1e-5
+    assert e_err <= TOL32
     assert abs(d["rho_peak"] - rho_ref) <= 1e-3 * rho_ref
