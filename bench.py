@@ -37,11 +37,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def variant_sweep(inp, prec, steps=3):
+    """per-variant kernel times (ms): the numbers the autotuner chooses from"""
+    from mpc_amd import Propagator
+    out = {}
+    p = Propagator(inp, prec)
+    for tv in range(2):
+        for av in range(4):
+            p.upload(inp.A)
+            p.calibrate()
+            p.set_variants(tv, av)
+            p.set_timing(True)
+            p.run(0, steps)
+            out[f"t{tv}a{av}"] = [round(p.kernel_ms(i), 4) for i in range(3)]
+            p.set_timing(False)
+    p.close()
+    log(f"[bench] {prec} variant sweep (t-line, x-line, y-line ms): {out}")
+    return out
+
+
 def timed_run(inp, prec, warmup, steps, with_kernel_timing=True):
     from mpc_amd import Propagator
     p = Propagator(inp, prec)
     p.upload(inp.A)
     p.calibrate()
+    variants = p.autotune()
+    log(f"[bench] {prec}: autotuned variants (t-line, axis) = {variants}")
     t0 = time.perf_counter()
     p.run(0, warmup)
     log(f"[bench] {prec}: warmup {warmup} steps {time.perf_counter() - t0:.2f}s")
@@ -61,7 +82,7 @@ def timed_run(inp, prec, warmup, steps, with_kernel_timing=True):
         log(f"[bench] {prec}: kernel ms t/x/y = {kms}")
     bytes_field = p.inp.A.size * (8 if prec == "f32" else 16)
     p.close()
-    return steps / dt, dt, A, kms, bytes_field
+    return steps / dt, dt, A, kms, bytes_field, variants
 
 
 def cpu_baseline(inp, max_seconds=30.0):
@@ -101,6 +122,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
     ap.add_argument("--nx", type=int, default=256, help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
@@ -119,8 +141,11 @@ def main(argv=None):
     inp = make_inputs(wl)
     log(f"[bench] inputs {wl.nx}x{wl.nx}x{wl.nt} generated in {time.perf_counter() - t0:.1f}s")
 
-    sps64, dt64, A64, kms64, _ = timed_run(inp, "f64", a.warmup, a.steps)
-    sps32, dt32, A32, kms32, bf32 = timed_run(inp, "f32", a.warmup, a.steps)
+    sweep = {}
+    if a.sweep_variants:
+        sweep = {"f32": variant_sweep(inp, "f32"), "f64": variant_sweep(inp, "f64")}
+    sps64, dt64, A64, kms64, _, var64 = timed_run(inp, "f64", a.warmup, a.steps)
+    sps32, dt32, A32, kms32, bf32, var32 = timed_run(inp, "f32", a.warmup, a.steps)
     rel = float(np.linalg.norm((A32 - A64).ravel()) / np.linalg.norm(A64.ravel()))
     log(f"[bench] fp32 vs fp64 rel L2 after {a.warmup + a.steps} steps: {rel:.3e}")
     del A32, A64
@@ -151,10 +176,13 @@ def main(argv=None):
                    "plain_fp64_steps_per_s": round(sps64, 3),
                    "fp32_over_fp64": round(sps32 / sps64, 3),
                    "fp32_vs_fp64_rel_l2": rel,
-                   "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)}},
+                   "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)},
+                   "autotuned_variants": {"f32": var32, "f64": var64}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if sweep:
+        out["variant_sweep_ms"] = sweep
     print(json.dumps(out))
     return 0
 

@@ -100,6 +100,15 @@ int  mpc_diagnostics(mpc_ctx *ctx, mpc_diag *d);
 int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int max_steps,
                       int *accepted, double *errs, double *dzs);
 
+/* FFT/tile autotuner: times every kernel variant (FFT radix plan, tile
+ * width, elements per thread) on a scratch copy of the resident field and
+ * keeps the fastest in the context (call after mpc_upload).  The variants
+ * compute the same transform; the choice only changes speed.  The
+ * MPC_TLINE_VARIANT / MPC_AXIS_VARIANT environment variables pin them. */
+int  mpc_autotune(mpc_ctx *ctx);
+int  mpc_get_variants(mpc_ctx *ctx, int *tline_variant, int *axis_variant);
+int  mpc_set_variants(mpc_ctx *ctx, int tline_variant, int axis_variant);
+
 /* Timing of the most recent mpc_run: average device time per launch of each
  * sweep kernel, measured with HIP events on the library's stream.
  * which: 0 = t-line, 1 = x-line, 2 = y-line.  Returns ms, or <0. */

@@ -161,6 +161,18 @@ class Propagator:
         return {"z": zc.value, "dzz": dc.value, "signmem": sm.value, "accepted": n,
                 "errs": errs[:n], "dzs": dzs[:n]}
 
+    def autotune(self) -> tuple:
+        check(lib().mpc_autotune(self.ctx), "mpc_autotune")
+        return self.variants()
+
+    def variants(self) -> tuple:
+        tv, av = C.c_int(), C.c_int()
+        check(lib().mpc_get_variants(self.ctx, C.byref(tv), C.byref(av)), "mpc_get_variants")
+        return tv.value, av.value
+
+    def set_variants(self, tline: int, axis: int) -> None:
+        check(lib().mpc_set_variants(self.ctx, tline, axis), "mpc_set_variants")
+
     def set_timing(self, on: bool) -> None:
         check(lib().mpc_set_timing(self.ctx, 1 if on else 0), "mpc_set_timing")
 

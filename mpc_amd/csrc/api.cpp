@@ -93,10 +93,18 @@ static int build_pk(mpc_ctx *c) {
     k.K4 = k0 / (p.n0 * p.rho_c) * p.w0;
     k.K5 = p.w0 * p.w0;
     k.w0 = p.w0;
+    k.ve_fac = std::sqrt(k.K1 / k.K2) / p.w0;
     k.I1e = I1 * EL0;
     k.Kon = p.deltat / T0;
     k.deltat = p.deltat;
     k.rho_nt = p.rho_nt;
+    k.zf = 2.0 * ZW0 / NC;
+    k.cs = k.zf * k.hbar_w0;
+    k.kt = 1.0 / (k.K3 * FCTR * k.ve_fac);
+    k.ke1 = k.kon3 / (3.0 * FCTR * k.inv_ef);
+    k.ke2 = k.kon4 / (3.0 * FCTR * k.inv_ef);
+    k.L1 = std::log(k.kon3 / (FCTR * k.inv_ef));
+    k.L2 = std::log(k.kon4 / (FCTR * k.inv_ef));
     return 0;
 }
 
@@ -197,6 +205,8 @@ extern "C" int mpc_create(const mpc_params *p, int precision, mpc_ctx **out) {
     }
     if (!rc) rc = ctx_set_params(c, p);
     if (rc) { mpc_destroy(c); return rc; }
+    if (const char *e = getenv("MPC_TLINE_VARIANT")) c->tvar = std::max(0, std::min(tline_variants() - 1, atoi(e)));
+    if (const char *e = getenv("MPC_AXIS_VARIANT")) c->avar = std::max(0, std::min(axis_variants() - 1, atoi(e)));
     *out = c;
     return 0;
 }
@@ -317,7 +327,7 @@ template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, b
         if (mirror) { a.M = (const cx<T> *)c->mirror; a.m_line = c->nx; a.m_outer = 1; }
     }
     tic(c, 1 + axis);
-    const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx, c->stream);
+    const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx, c->avar, c->stream);
     toc(c, 1 + axis);
     return rc;
 }
@@ -341,6 +351,7 @@ template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q
     const double s2 = c->ascale * c->ascale;   /* the field holds A / ascale */
     a.pk = c->pk;
     a.pk.hbar_w0 = c->pk.hbar_w0 * s2;
+    a.pk.cs = c->pk.cs * s2;
     a.dz = (T)dz;
     a.gamma = (T)(c->p.gamma2 * s2);
     a.sst_coef = (T)(c->p.gamma2 / c->p.w0 * s2);
@@ -352,7 +363,7 @@ template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q
     const bool plasma = do_nl && c->p.plasm == 1;
     a.rho_block = plasma ? c->rho_block : nullptr;
     tic(c, 0);
-    const int rc = launch_tline<T>(a, c->nt, plasma, c->stream);
+    const int rc = launch_tline<T>(a, c->nt, plasma, c->tvar, c->stream);
     toc(c, 0);
     return rc;
 }
@@ -653,6 +664,82 @@ extern "C" int mpc_run_adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem
 }
 
 /* ------------------------------------------------------------------------ */
+/* FFT / tile autotuner                                                      */
+/* ------------------------------------------------------------------------ */
+/* times every kernel variant (median of 3 launches) on a scratch copy of the
+ * resident field and keeps the fastest; the choice only changes speed */
+template <class T> static int autotune(mpc_ctx *c) {
+    const size_t bytes = c->esz * (size_t)c->N;
+    if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
+    HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, c->stream));
+    void *keep = c->field;
+    const int keep_t = c->tvar, keep_a = c->avar, keep_timing = c->timing;
+    c->field = c->aux1;
+    c->timing = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time3 = [&](auto &&launch) -> float {
+        float best[3];
+        for (int r = 0; r < 3; r++) {
+            (void)hipEventRecord(e0, c->stream);
+            if (launch() != 0) return 1e30f;
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 1e30f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            best[r] = ms;
+        }
+        std::sort(best, best + 3);
+        return best[1];
+    };
+    const double dz = c->p.dzz;
+    const cx<T> *D = get_table<T>(c, 'x', dz);
+    const cx<T> *P = get_table<T>(c, 't', dz);
+    int rc = (!D || !P) ? MPC_ENOMEM : 0;
+    float bestA = 1e30f, bestT = 1e30f;
+    int va = keep_a, vt = keep_t;
+    for (int v = 0; !rc && v < axis_variants(); v++) {
+        c->avar = v;
+        const float ms = time3([&] { return sweep_axis<T>(c, 0, D, false); }) +
+                         time3([&] { return sweep_axis<T>(c, 1, D, false); });
+        if (ms < bestA) { bestA = ms; va = v; }
+    }
+    for (int v = 0; !rc && v < tline_variants(); v++) {
+        c->tvar = v;
+        const float ms = time3([&] { return sweep_t<T>(c, P, nullptr, true, true, false, dz); });
+        if (ms < bestT) { bestT = ms; vt = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->field = keep;
+    c->timing = keep_timing;
+    c->avar = rc ? keep_a : va;
+    c->tvar = rc ? keep_t : vt;
+    if (rc) return rc;
+    c->tuned = 1;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mpc_autotune(mpc_ctx *c) {
+    if (!c) return MPC_EINVAL;
+    return c->prec == MPC_F32 ? autotune<float>(c) : autotune<double>(c);
+}
+extern "C" int mpc_get_variants(mpc_ctx *c, int *tv, int *av) {
+    if (!c) return MPC_EINVAL;
+    if (tv) *tv = c->tvar;
+    if (av) *av = c->avar;
+    return 0;
+}
+extern "C" int mpc_set_variants(mpc_ctx *c, int tv, int av) {
+    if (!c || tv < 0 || tv >= tline_variants() || av < 0 || av >= axis_variants()) return MPC_EINVAL;
+    c->tvar = tv;
+    c->avar = av;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* legacy entry points (propagation.h / plasma.h)                            */
 /* ------------------------------------------------------------------------ */
 struct mpc_plan_s {
@@ -660,7 +747,9 @@ struct mpc_plan_s {
         int dims[3];
         int refs;
         mpc_ctx *ctx;
-        std::string wisdom;
+        std::string wisdom;   /* "<cpath><Nx>": the kernel-variant cache */
+        int readin;
+        int tvar, avar;       /* from the cache, or -1 until tuned */
     } *sh;
     int axis, dir;
 };
@@ -690,19 +779,38 @@ extern "C" void createPlans(int readin, int *dims, const char *cpath, fftw_plan 
         p->dir = dr[i];
         *outs[i] = p;
     }
-    /* the autotuner cache ("wisdom", propagation.c:540-552,609-611) */
-    if (!sh->wisdom.empty()) {
-        FILE *f = readin ? fopen(sh->wisdom.c_str(), "r") : nullptr;
+    /* the autotuner cache, read like FFTW wisdom (propagation.c:540-552);
+     * it is written after the first tuning (:609-611) */
+    sh->readin = readin;
+    sh->tvar = sh->avar = -1;
+    if (readin && !sh->wisdom.empty()) {
+        FILE *f = fopen(sh->wisdom.c_str(), "r");
         if (f) {
+            int t = -1, a = -1;
+            if (fscanf(f, "mpc-kernel-variants tline=%d axis=%d", &t, &a) == 2) { sh->tvar = t; sh->avar = a; }
+            else printf("Importing wisdom failed, creating new one to save.\n");
             fclose(f);
-        } else {
-            f = fopen(sh->wisdom.c_str(), "w");
-            if (f) {
-                fprintf(f, "mpc-fft-plan nx=%d ny=%d nt=%d t-radices=16,16,8 xy-radices=16,16\n", dims[0], dims[1], dims[2]);
-                fclose(f);
-            }
         }
     }
+}
+
+/* apply the cached kernel variants, or tune on the uploaded field and cache */
+static int legacy_tune(mpc_plan_s::Shared *sh) {
+    mpc_ctx *c = sh->ctx;
+    if (c->tuned) return 0;
+    if (sh->tvar >= 0 && mpc_set_variants(c, sh->tvar, sh->avar) == 0) { c->tuned = 1; return 0; }
+    RCHK(mpc_autotune(c));
+    sh->tvar = c->tvar;
+    sh->avar = c->avar;
+    if (!sh->wisdom.empty()) {
+        FILE *f = fopen(sh->wisdom.c_str(), "w");
+        if (f) {
+            fprintf(f, "mpc-kernel-variants tline=%d axis=%d nx=%d ny=%d nt=%d\n", c->tvar, c->avar, sh->dims[0],
+                    sh->dims[1], sh->dims[2]);
+            fclose(f);
+        }
+    }
+    return 0;
 }
 
 extern "C" void mpc_destroy_plan(fftw_plan p) {
@@ -763,6 +871,7 @@ extern "C" void ssfmprop(cufftDoubleComplex *A, int *dims, int sst_on, double dz
     mpc_ctx *c = nullptr;
     int rc = legacy_ctx(pTfft, p, &c);
     if (!rc) rc = mpc_upload(c, A);
+    if (!rc) rc = legacy_tune(pTfft->sh);
     if (!rc) rc = mpc_step(c, dzz, z, signmem);
     if (!rc && change == 1) {
         /* propagation.c:334-341 (the absorption branch cannot trigger once
@@ -861,6 +970,7 @@ extern "C" void ioniz(cufftDoubleComplex *A, int *dims, double w0, double deltat
          * transpose used for fields brings both back to A[t][y][x] order */
         PlasmaK pk = c->pk;
         pk.hbar_w0 *= c->ascale * c->ascale;
+        pk.cs *= c->ascale * c->ascale;
         rc = launch_ioniz<double>((const cx<double> *)c->field, c->nt, c->nxy, pk, (cx<double> *)out, c->stream);
     }
     if (!rc) rc = launch_to_reference<double>((const cx<double> *)out, c->staging, c->nx, c->ny, c->nt, 1.0, c->stream);

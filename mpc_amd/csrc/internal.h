@@ -20,13 +20,23 @@
 
 namespace mpc {
 
-/* FP64 constants of the plasma model (plasma.c:141-212), evaluated once */
+/* FP64 constants of the plasma model (plasma.c:141-212), evaluated once.
+ * With s = sqrt(2 ZW0 I / NC) (so E0 = FCTR s) every per-element quantity
+ * is a product with s or 1/s:
+ *   tau_c = kt / s,  sigma = K4 tau_c / (1 + K5 tau_c^2),
+ *   ADK exponent Kon3 / (3 Ea) = ke1 / s,  log(Kon3 / Ea) = L1 + log(1/s). */
 struct PlasmaK {
+    double zf;           /* 2 ZW0 / NC */
+    double cs;           /* zf * HBAR w0 (scaled by the field scale^2 at launch) */
+    double kt;           /* 1 / (K3 FCTR sqrt(K1/K2) / w0) */
+    double ke1, ke2;     /* Kon3 / (3 FCTR / EF), Kon4 / (3 FCTR / EF) */
+    double L1, L2;       /* log(Kon3 EF / FCTR), log(Kon4 EF / FCTR) */
     double hbar_w0;      /* HBAR * w0 */
     double inv_ef;       /* 1 / EF */
     double kon1, kon2, kon3, kon4, kon5, kon6;
     double K1, K2, K3, K4, K5;
     double w0;
+    double ve_fac;       /* sqrt(K1 / K2) / w0: quiver velocity per unit field */
     double I1e;          /* I1[eV] * EL0 */
     double Kon;          /* deltat / T0 */
     double deltat;
@@ -58,8 +68,10 @@ template <class T> struct AArgs {
 };
 
 /* launchers (kernels.hip) */
-template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, hipStream_t s);
-template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, hipStream_t s);
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s);
+template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s);
+int tline_variants(void);
+int axis_variants(void);
 template <class T> int launch_to_internal(const double *src, cx<T> *dst, int nx, int ny, int nt, double scale,
                                           hipStream_t s);
 template <class T> int launch_to_reference(const cx<T> *src, double *dst, int nx, int ny, int nt, double scale,
@@ -109,6 +121,8 @@ struct mpc_ctx_s {
     mpc::PlasmaK pk{};
     std::map<std::string, void *> tables;   /* table cache keyed by kind+coef */
     int domain_freq = 0;             /* resident field is F_t(A) (unnormalised) */
+    int tvar = 0, avar = 0;          /* kernel variants (autotuner / MPC_*_VARIANT) */
+    int tuned = 0;
     /* the resident field holds A / ascale; ascale is a power of two chosen
      * at upload so |A / ascale| <= 1 (photon-flux amplitudes reach 1e18 and
      * their SST products would overflow FP32).  Nonlinear coefficients are

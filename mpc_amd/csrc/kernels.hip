@@ -52,80 +52,74 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[PL::E], int tid, cx<T> *lds,
     DevFftP<T, N, PL::E, DIR, PAD, typename PL::R>::run(v, tid, lds, ls, tw);
 }
 
-/* t-line geometry: plan, LDS padding and lines per 256-thread block */
-template <int NT> struct TGeo {
-    using PL = TPlan<NT>;
+/* t-line geometry: plan, LDS padding and lines per 256-thread block.  The
+ * LDS pad slot every E elements matches the stride of the first Stockham
+ * stage's writes. */
+template <int NT_, class PL_ = TPlan<NT_>> struct TGeo {
+    static constexpr int NT = NT_;
+    using PL = PL_;
     static constexpr int E = PL::E, TPL = NT / E;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int THREADS = LPB * TPL;
-    static constexpr int PAD = 8;                       /* LDS pad slot every 8 elements */
+    static constexpr int PAD = E;
     static constexpr int FFT_ELEMS = NT + NT / PAD;
 };
 
 /* ---------------------------------------------------------------------- */
 /* plasma model pieces (FP64 in both precisions)                           */
 /* ---------------------------------------------------------------------- */
-struct PEl {
-    double b, a, c;      /* W1*Kon, avalanche*dt, W2*Kon */
-    double sig, tau;
-};
-
-/* ADK rate Kon_a (Kon_b/E)^Kon_c exp(-Kon_b/(3E)) in log form; exactly 0
- * where the exponential underflows in double precision (plasma.c:55-56). */
-__device__ __forceinline__ double adk_rate(double ka, double kb, double kc, double Ea) {
-    if (ka == 0.0) return 0.0;
-    const double e = kb / (3.0 * Ea);
-    if (e > 760.0) return 0.0;
-    const double w = ka * exp(kc * log(kb / Ea) - e);
+/* ADK rate Kon_a (Kon_b/Ea)^Kon_c exp(-Kon_b/(3 Ea)) (plasma.c:55-56) in log
+ * form with e = Kon_b/(3 Ea) = ke * (1/s) and log(Kon_b/Ea) = L + log(1/s);
+ * exactly 0 where the exponential underflows in double precision. */
+__device__ __forceinline__ double adk_rate(double ka, double ke, double L, double kc, double invs) {
+    const double e = ke * invs;
+    if (ka == 0.0 || e > 760.0) return 0.0;
+    const double w = ka * exp(kc * (L + log(invs)) - e);
     return (w != w) ? 0.0 : w;
 }
 
-/* per-element plasma coefficients from |A|^2 (plasma.c:133-212, corrected
- * arithmetic: quiver velocity at w0, avalanche rate in SI units) */
-__device__ __forceinline__ PEl plasma_el(double a2, const PlasmaK &k, bool rates) {
-    PEl r;
+/* field scale s = sqrt(2 ZW0 I / NC) from |A|^2, with I floored at 1e-30
+ * (plasma.c:135-138, :147) */
+__device__ __forceinline__ double plasma_s(double a2, const PlasmaK &k, double *Iout) {
     double I = a2 * k.hbar_w0;
     if (I == 0.0) I = 1e-30;
-    const double E0 = sqrt(2.0 * 377.0 * I / 1.0) * 0.89;
-    const double Ea = fabs(E0) * k.inv_ef;
-    const double ve = sqrt(k.K1 * (E0 * E0) / (k.K2 * (k.w0 * k.w0)));
-    const double tc = 1.0 / (k.K3 * ve);
-    r.tau = tc;
-    r.sig = k.K4 * tc / (1.0 + k.K5 * (tc * tc));
-    if (rates) {
-        const double W1 = adk_rate(k.kon1, k.kon3, k.kon5, Ea);
-        const double W2 = adk_rate(k.kon2, k.kon4, k.kon6, Ea);
-        const double Wava = r.sig * I / k.I1e;
-        r.b = W1 * k.Kon;
-        r.a = Wava * k.deltat;
-        r.c = W2 * k.Kon;
-    } else {
-        r.b = r.a = r.c = 0.0;
-    }
-    return r;
+    *Iout = I;
+    return sqrt(k.zf * I);
+}
+/* collision time and inverse-bremsstrahlung cross-section from 1/s
+ * (plasma.c:207-210, quiver velocity at w0) */
+__device__ __forceinline__ double plasma_sigma(double invs, const PlasmaK &k, double *tau) {
+    const double tc = k.kt * invs;
+    *tau = tc;
+    return k.K4 * tc / (1.0 + k.K5 * (tc * tc));
+}
+/* -expm1(-R): a 5-term series below 1e-4 (truncation < 1e-20 relative) */
+__device__ __forceinline__ double one_minus_exp_neg(double R) {
+    if (R < 1e-4) return R * (1.0 + R * (-0.5 + R * (1.0 / 6.0 + R * (-1.0 / 24.0 + R * (1.0 / 120.0)))));
+    return -expm1(-R);
 }
 
 /* plasma arrays: one pad slot per E-element segment, so the per-thread
  * consecutive segments of the scan start in different LDS banks */
-template <int NT> __device__ __forceinline__ int padp(int i) { return i + i / TGeo<NT>::E; }
+template <class G> __device__ __forceinline__ int padp(int i) { return i + i / G::E; }
 
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
- * f_k(R) = M[k] R + C[k]; the result R_k = f_k(R_{k+1}) with R_NT = 0 is
- * written over M[k].  Thread tid owns the consecutive segment
+ * f_k(R) = M[k] R + C[k]; the result R_k = f_k(R_{k+1}) with R_NT = 0 (or,
+ * with U, 1 - exp(-R_k)) is written over M[k].  Thread tid owns the consecutive segment
  * [tid*E, tid*E + E); segment maps are composed serially, combined across
  * lanes with a shuffle-based suffix scan and across wavefronts through LDS.
  */
-template <int NT>
+template <class G, bool U = false>
 __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const double *C, double *scr) {
-    constexpr int E = TGeo<NT>::E, TPL = NT / E;
+    constexpr int E = G::E, TPL = G::NT / E;
     constexpr int W = TPL < 64 ? TPL : 64;
     constexpr int NW = (TPL + 63) / 64;
     const int lo = tid * E;
     double gm = 1.0, gc = 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = padp<NT>(lo + i);
+        const int k = padp<G>(lo + i);
         const double mk = M[k], ck = C[k];
         gc = mk * gc + ck;
         gm = mk * gm;
@@ -155,15 +149,15 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const dou
     double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = padp<NT>(lo + i);
+        const int k = padp<G>(lo + i);
         R = M[k] * R + C[k];
-        M[k] = R;
+        M[k] = U ? one_minus_exp_neg(R) : R;   /* U: store u = 1 - exp(-R) instead of R */
     }
     __syncthreads();
 }
 
-template <int NT> struct PlasmaLds {
-    static constexpr int E = TGeo<NT>::E, TPL = NT / E, NW = (TPL + 63) / 64;
+template <class G> struct PlasmaLds {
+    static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
     static constexpr int ARR = NT + NT / E;            /* padded doubles per array */
     static constexpr int SCR = 2 * NW + TPL;
     static constexpr int BYTES = (2 * ARR + SCR) * 8;
@@ -178,54 +172,63 @@ template <int NT> struct PlasmaLds {
  * Element k = tid + TPL*m of the line is held by thread tid in slot m.
  * sink(m, F, tau) receives the result per element.
  */
-template <class T, int NT, class Sink>
-__device__ __forceinline__ void plasma_line(const cx<T> (&A)[TGeo<NT>::E], int tid, double *lds,
+template <class T, class G, class Sink>
+__device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, double *lds,
                                             const PlasmaK &pk, double &rho_max, Sink sink) {
-    using L = PlasmaLds<NT>;
+    using L = PlasmaLds<G>;
+    constexpr int NT = G::NT;
     constexpr int E = L::E, TPL = L::TPL;
     double *Lm = lds, *Lc = lds + L::ARR, *scr = lds + 2 * L::ARR;
-    double cc[E];
     int anyc = 0;
+    double invs[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
-        const PEl e = plasma_el(a2, pk, true);
-        Lm[padp<NT>(k)] = 1.0 - e.a;
-        Lc[padp<NT>(k)] = e.b * (1.0 - e.a) + e.a;
-        cc[m] = e.c;
-        anyc |= (e.c != 0.0);
+        double I;
+        const double sv = plasma_s(a2, pk, &I);
+        invs[m] = 1.0 / sv;
+        double tc;
+        const double sig = plasma_sigma(invs[m], pk, &tc);
+        const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, invs[m]) * pk.Kon;
+        const double a = sig * I / pk.I1e * pk.deltat;   /* avalanche, SI units */
+        Lm[padp<G>(k)] = 1.0 - a;
+        Lc[padp<G>(k)] = b * (1.0 - a) + a;
+        anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= 760.0);
     }
     __syncthreads();
-    affine_suffix_scan<NT>(tid, Lm, Lc, scr);         /* Lm <- R */
+    affine_suffix_scan<G, true>(tid, Lm, Lc, scr);    /* Lm <- u = 1 - exp(-R) */
     const int block_anyc = __syncthreads_or(anyc);
     double u[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) u[m] = -expm1(-Lm[padp<NT>(tid + TPL * m)]);
+    for (int m = 0; m < E; ++m) u[m] = Lm[padp<G>(tid + TPL * m)];
     if (block_anyc) {
+        /* second scan: ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1} */
+        double cp[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int k = tid + TPL * m;
-            const double un = (k + 1 < NT) ? -expm1(-Lm[padp<NT>(k + 1)]) : 0.0;
-            Lc[padp<NT>(k)] = 1.0 - cc[m];   /* M' */
-            cc[m] = cc[m] * un;           /* C' */
+            const double un = (k + 1 < NT) ? Lm[padp<G>(k + 1)] : 0.0;
+            const double c = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, invs[m]) * pk.Kon;
+            Lc[padp<G>(k)] = 1.0 - c;     /* M' */
+            cp[m] = c * un;               /* C' */
         }
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) Lm[padp<NT>(tid + TPL * m)] = cc[m];
+        for (int m = 0; m < E; ++m) Lm[padp<G>(tid + TPL * m)] = cp[m];
         __syncthreads();
-        affine_suffix_scan<NT>(tid, Lc, Lm, scr);     /* Lc <- ions2 */
+        affine_suffix_scan<G>(tid, Lc, Lm, scr);      /* Lc <- ions2 */
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
-        const double i2 = block_anyc ? Lc[padp<NT>(k)] : 0.0;
+        const double i2 = block_anyc ? Lc[padp<G>(k)] : 0.0;
         const double i1 = u[m] - i2;
         const double fr = i1 + 2.0 * i2;
-        const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
-        const PEl e = plasma_el(a2, pk, false);
+        double tc;
+        const double sig = plasma_sigma(invs[m], pk, &tc);
         rho_max = fmax(rho_max, fr);
-        sink(m, fr * e.sig, e.tau);
+        sink(m, fr * sig, tc);
     }
     __syncthreads();
 }
@@ -233,11 +236,10 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[TGeo<NT>::E], int t
 /* ---------------------------------------------------------------------- */
 /* t-line kernel                                                           */
 /* ---------------------------------------------------------------------- */
-template <class T, int NT, bool PLASMA> struct TCfg {
-    using G = TGeo<NT>;
+template <class T, class G, bool PLASMA> struct TCfg {
     static constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB, THREADS = G::THREADS;
     static constexpr int FFT_BYTES = G::FFT_ELEMS * (int)sizeof(cx<T>);
-    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<NT>::BYTES : 0;
+    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<G>::BYTES : 0;
     static constexpr int LINE_BYTES = ((FFT_BYTES > PL_BYTES ? FFT_BYTES : PL_BYTES) + 15) / 16 * 16;
 };
 
@@ -311,9 +313,11 @@ template <class T> __device__ __forceinline__ T exp_t(T x);
 template <> __device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
 
-template <class T, int NT, bool PLASMA>
-__global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<T> a) {
-    using C = TCfg<T, NT, PLASMA>;
+template <class T, class G, bool PLASMA>
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS)) k_tline(TArgs<T> a) {
+    using C = TCfg<T, G, PLASMA>;
+    using PL = typename G::PL;
+    constexpr int NT = G::NT;
     constexpr int E = C::E, TPL = C::TPL;
     __shared__ __align__(16) unsigned char smem[C::LPB * C::LINE_BYTES];
     const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
@@ -325,7 +329,7 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
     cx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = g[tid + TPL * m];
-    if (!a.in_freq) line_fft<T, NT, TPlan<NT>, -1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
+    if (!a.in_freq) line_fft<T, NT, PL, -1, G::PAD>(v, tid, L, 1, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
 
@@ -337,9 +341,9 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
                 const T wk = a.wfft[tid + TPL * m];
                 d[m] = mk(wk * v[m].im, -wk * v[m].re);   /* -i omega A(omega) */
             }
-            line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(d, tid, L, 1, a.tw);   /* dA */
+            line_fft<T, NT, PL, +1, G::PAD>(d, tid, L, 1, a.tw);   /* dA */
         }
-        line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);       /* A  */
+        line_fft<T, NT, PL, +1, G::PAD>(v, tid, L, 1, a.tw);       /* A  */
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
@@ -357,7 +361,7 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
         if constexpr (PLASMA) {
             double rmax = 0.0;
             const double half_rho = 0.5 * a.pk.rho_nt;
-            plasma_line<T, NT>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
+            plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
                                [&](int m, double F, double tau) {
                                    /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
                                    const double z = half_rho * F;
@@ -387,48 +391,58 @@ __global__ void __launch_bounds__((TCfg<T, NT, PLASMA>::THREADS)) k_tline(TArgs<
             const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
             v[m] = mk(er * z.re, er * z.im);
         }
-        line_fft<T, NT, TPlan<NT>, -1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
+        line_fft<T, NT, PL, -1, G::PAD>(v, tid, L, 1, a.tw);
     }
     if (a.Q) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
     }
-    if (a.out_time) line_fft<T, NT, TPlan<NT>, +1, TGeo<NT>::PAD>(v, tid, L, 1, a.tw);
+    if (a.out_time) line_fft<T, NT, PL, +1, G::PAD>(v, tid, L, 1, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
 }
 
 bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
+int tline_variants(void) { return 2; }
 
-template <class T, int NT, bool PL>
+template <class T, class G, bool PL>
 static int tline_go(const TArgs<T> &a, hipStream_t s) {
-    using C = TCfg<T, NT, PL>;
+    using C = TCfg<T, G, PL>;
     if (a.nlines % C::LPB) return MPC_EINVAL;
     const long nb = a.nlines / C::LPB;
-    hipLaunchKernelGGL((k_tline<T, NT, PL>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
+    hipLaunchKernelGGL((k_tline<T, G, PL>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
-
-template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, hipStream_t s) {
+/* variant 0: 8 elements per thread (TPlan); 1: 16 per thread (Plan) */
+template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, int variant, hipStream_t s) {
+    if (variant == 1)
+        return plasma ? tline_go<T, TGeo<NT, Plan<NT>>, true>(a, s) : tline_go<T, TGeo<NT, Plan<NT>>, false>(a, s);
+    return plasma ? tline_go<T, TGeo<NT>, true>(a, s) : tline_go<T, TGeo<NT>, false>(a, s);
+}
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s) {
     switch (nt) {
-    case 512:  return plasma ? tline_go<T, 512, true>(a, s)  : tline_go<T, 512, false>(a, s);
-    case 1024: return plasma ? tline_go<T, 1024, true>(a, s) : tline_go<T, 1024, false>(a, s);
-    case 2048: return plasma ? tline_go<T, 2048, true>(a, s) : tline_go<T, 2048, false>(a, s);
-    case 4096: return plasma ? tline_go<T, 4096, true>(a, s) : tline_go<T, 4096, false>(a, s);
+    case 512:  return tline_v<T, 512>(a, plasma, variant, s);
+    case 1024: return tline_v<T, 1024>(a, plasma, variant, s);
+    case 2048: return tline_v<T, 2048>(a, plasma, variant, s);
+    case 4096: return tline_v<T, 4096>(a, plasma, variant, s);
     default: return MPC_EUNSUPP;
     }
 }
-template int launch_tline<double>(const TArgs<double> &, int, bool, hipStream_t);
-template int launch_tline<float>(const TArgs<float> &, int, bool, hipStream_t);
+template int launch_tline<double>(const TArgs<double> &, int, bool, int, hipStream_t);
+template int launch_tline<float>(const TArgs<float> &, int, bool, int, hipStream_t);
 
-/* t-line kernel block geometry, for callers that size per-block buffers */
+/* lines per block are the same for every t-line variant: (256 / TPL) with
+ * TPL = NT / E, and every variant's rho_block is sized by the largest grid */
+/* t-line kernel block geometry: the smallest lines-per-block over the
+ * variants, so per-block buffers sized by it fit every variant's grid */
 int tline_lines_per_block(int nt, bool plasma, int prec) {
     (void)plasma; (void)prec;
+    auto mn = [](int a, int b) { return a < b ? a : b; };
     switch (nt) {
-    case 512: return TGeo<512>::LPB;
-    case 1024: return TGeo<1024>::LPB;
-    case 2048: return TGeo<2048>::LPB;
-    case 4096: return TGeo<4096>::LPB;
+    case 512: return mn(TGeo<512>::LPB, TGeo<512, Plan<512>>::LPB);
+    case 1024: return mn(TGeo<1024>::LPB, TGeo<1024, Plan<1024>>::LPB);
+    case 2048: return mn(TGeo<2048>::LPB, TGeo<2048, Plan<2048>>::LPB);
+    case 4096: return mn(TGeo<4096>::LPB, TGeo<4096, Plan<4096>>::LPB);
     default: return 1;
     }
 }
@@ -439,14 +453,22 @@ int tline_lines_per_block(int nt, bool plasma, int prec) {
 template <class T> struct ATile { static constexpr int G = 256 / (int)sizeof(cx<T>); };  /* 256 B rows */
 int axis_tile(int precision) { return precision == MPC_F32 ? ATile<float>::G : ATile<double>::G; }
 
-template <class T, int N> struct ACfg {
-    static constexpr int E = Plan<N>::E, TPL = N / E, G = ATile<T>::G;
+/* 8-elements-per-thread plans for the transverse lines */
+template <int N> struct APlan8 { using P = Plan<N>; static constexpr int E = P::E; using R = typename P::R; };
+template <> struct APlan8<512> { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
+template <> struct APlan8<256> { static constexpr int E = 8; using R = Radices<8, 8, 4>; };
+template <> struct APlan8<128> { static constexpr int E = 8; using R = Radices<8, 8, 2>; };
+template <> struct APlan8<64>  { static constexpr int E = 8; using R = Radices<8, 8>; };
+template <> struct APlan8<16>  { static constexpr int E = 8; using R = Radices<8, 2>; };
+
+template <class T, int N, class PL, int G_> struct ACfg {
+    static constexpr int E = PL::E, TPL = N / E, G = G_;
     static constexpr int THREADS = G * TPL;
 };
 
-template <class T, int N>
-__global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
-    using C = ACfg<T, N>;
+template <class T, int N, class PL, int G_>
+__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T> a) {
+    using C = ACfg<T, N, PL, G_>;
     constexpr int E = C::E, TPL = C::TPL, G = C::G;
     __shared__ __align__(16) cx<T> lds[N * G];
     const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
@@ -456,10 +478,10 @@ __global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
     cx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = base[(long)(tid + TPL * m) * a.s_line];
-    line_fft<T, N, Plan<N>, -1>(v, tid, lds + gcol, G, a.tw);
+    line_fft<T, N, PL, -1>(v, tid, lds + gcol, G, a.tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.D[tid + TPL * m]);
-    line_fft<T, N, Plan<N>, +1>(v, tid, lds + gcol, G, a.tw);
+    line_fft<T, N, PL, +1>(v, tid, lds + gcol, G, a.tw);
     if (a.M) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
@@ -469,27 +491,39 @@ __global__ void __launch_bounds__((ACfg<T, N>::THREADS)) k_axis(AArgs<T> a) {
 }
 
 bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+int axis_variants(void) { return 4; }
 
-template <class T, int N> static int axis_go(const AArgs<T> &a, int outer, hipStream_t s) {
-    using C = ACfg<T, N>;
-    if (a.nt % C::G) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_axis<T, N>), dim3((unsigned)(a.nt / C::G), (unsigned)outer), dim3(C::THREADS), 0, s, a);
+template <class T, int N, class PL, int G> static int axis_go(const AArgs<T> &a, int outer, hipStream_t s) {
+    using C = ACfg<T, N, PL, G>;
+    if (a.nt % G) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_axis<T, N, PL, G>), dim3((unsigned)(a.nt / G), (unsigned)outer), dim3(C::THREADS), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
-template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, hipStream_t s) {
+/* variants: 0: Plan, 256-B rows; 1: Plan, 128-B rows; 2: 8 per thread,
+ * 128-B rows; 3: 8 per thread, 256-B rows */
+template <class T, int N> static int axis_v(const AArgs<T> &a, int outer, int variant, hipStream_t s) {
+    constexpr int G = ATile<T>::G;
+    switch (variant) {
+    case 1: return axis_go<T, N, Plan<N>, G / 2>(a, outer, s);
+    case 2: return axis_go<T, N, APlan8<N>, G / 2>(a, outer, s);
+    case 3: return axis_go<T, N, APlan8<N>, G>(a, outer, s);
+    default: return axis_go<T, N, Plan<N>, G>(a, outer, s);
+    }
+}
+template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s) {
     switch (n) {
-    case 16:  return axis_go<T, 16>(a, outer, s);
-    case 32:  return axis_go<T, 32>(a, outer, s);
-    case 64:  return axis_go<T, 64>(a, outer, s);
-    case 128: return axis_go<T, 128>(a, outer, s);
-    case 192: return axis_go<T, 192>(a, outer, s);
-    case 256: return axis_go<T, 256>(a, outer, s);
-    case 512: return axis_go<T, 512>(a, outer, s);
+    case 16:  return axis_v<T, 16>(a, outer, variant, s);
+    case 32:  return axis_v<T, 32>(a, outer, variant, s);
+    case 64:  return axis_v<T, 64>(a, outer, variant, s);
+    case 128: return axis_v<T, 128>(a, outer, variant, s);
+    case 192: return axis_v<T, 192>(a, outer, variant, s);
+    case 256: return axis_v<T, 256>(a, outer, variant, s);
+    case 512: return axis_v<T, 512>(a, outer, variant, s);
     default: return MPC_EUNSUPP;
     }
 }
-template int launch_axis<double>(const AArgs<double> &, int, int, hipStream_t);
-template int launch_axis<float>(const AArgs<float> &, int, int, hipStream_t);
+template int launch_axis<double>(const AArgs<double> &, int, int, int, hipStream_t);
+template int launch_axis<float>(const AArgs<float> &, int, int, int, hipStream_t);
 
 /* ---------------------------------------------------------------------- */
 /* layout conversion: reference A[t][y][x] (FP64) <-> internal A[y][x][t]  */
@@ -717,9 +751,10 @@ template int launch_richardson<float>(cx<float> *, const cx<float> *, const cx<f
 /* legacy ioniz(): tau_c and F per element (internal layout outputs)       */
 /* ---------------------------------------------------------------------- */
 template <class T, int NT>
-__global__ void __launch_bounds__((TCfg<T, NT, true>::THREADS)) k_ioniz(const cx<T> *f, long nlines, PlasmaK pk,
+__global__ void __launch_bounds__((TCfg<T, TGeo<NT>, true>::THREADS)) k_ioniz(const cx<T> *f, long nlines, PlasmaK pk,
                                                                      cx<double> *out) {
-    using C = TCfg<T, NT, true>;
+    using G = TGeo<NT>;
+    using C = TCfg<T, G, true>;
     constexpr int E = C::E, TPL = C::TPL;
     __shared__ __align__(16) unsigned char smem[C::LPB * C::LINE_BYTES];
     const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
@@ -728,13 +763,13 @@ __global__ void __launch_bounds__((TCfg<T, NT, true>::THREADS)) k_ioniz(const cx
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = f[line * NT + tid + TPL * m];
     double rmax = 0.0;
-    plasma_line<T, NT>(v, tid, reinterpret_cast<double *>(smem + lslot * C::LINE_BYTES), pk, rmax,
+    plasma_line<T, G>(v, tid, reinterpret_cast<double *>(smem + lslot * C::LINE_BYTES), pk, rmax,
                        [&](int m, double Fv, double tv) {
                            out[line * NT + tid + TPL * m] = mk(tv, Fv);   /* (tau_c, F) */
                        });
 }
 template <class T, int NT> static int ioniz_go(const cx<T> *f, long nlines, const PlasmaK &pk, cx<double> *out, hipStream_t s) {
-    using C = TCfg<T, NT, true>;
+    using C = TCfg<T, TGeo<NT>, true>;
     if (nlines % C::LPB) return MPC_EINVAL;
     hipLaunchKernelGGL((k_ioniz<T, NT>), dim3((unsigned)(nlines / C::LPB)), dim3(C::THREADS), 0, s, f, nlines, pk, out);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;

@@ -174,3 +174,25 @@ def test_cfg3_fp32_precision_form(gpu_lib):
     assert err <= TOL32
     assert e_err <= TOL32
     assert abs(d["rho_peak"] - rho_ref) <= 1e-3 * rho_ref
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+def test_every_kernel_variant_matches_oracle(gpu_lib, prec, tol):
+    """the autotuner's choices (FFT plan, tile, elements per thread) change speed only"""
+    from mpc_amd import lib
+    inp, A0 = _case(64, 1024, plasm=1, sst_on=1, steps_per_pass=2, passes=2)
+    ref, _ = O.run_fixed(inp, A0, 0, 3)
+    outs = {}
+    for tv in range(2):
+        for av in range(4):
+            print(f"case variant tline={tv} axis={av} prec={prec}", flush=True)
+            with Propagator(inp, prec) as p:
+                p.upload(A0)
+                p.set_variants(tv, av)
+                p.run(0, 3)
+                outs[(tv, av)] = p.download()
+            assert O.rel_l2(outs[(tv, av)], ref) <= tol
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        tv, av = p.autotune()
+        assert 0 <= tv < 2 and 0 <= av < 4
