@@ -123,6 +123,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
+    ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
+                    help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
     ap.add_argument("--nx", type=int, default=256, help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
@@ -144,19 +146,26 @@ def main(argv=None):
     sweep = {}
     if a.sweep_variants:
         sweep = {"f32": variant_sweep(inp, "f32"), "f64": variant_sweep(inp, "f64")}
-    sps64, dt64, A64, kms64, _, var64 = timed_run(inp, "f64", a.warmup, a.steps)
-    sps32, dt32, A32, kms32, bf32, var32 = timed_run(inp, "f32", a.warmup, a.steps)
-    rel = float(np.linalg.norm((A32 - A64).ravel()) / np.linalg.norm(A64.ravel()))
-    log(f"[bench] fp32 vs fp64 rel L2 after {a.warmup + a.steps} steps: {rel:.3e}")
-    del A32, A64
+    sps64 = kms64 = var64 = rel = None
+    if a.prec in ("both", "f64"):
+        sps64, dt64, A64, kms64, bf64, var64 = timed_run(inp, "f64", a.warmup, a.steps)
+    if a.prec == "f64":                       # profiling run of the FP64 path alone
+        sps32, dt32, A32, kms32, bf32, var32 = sps64, dt64, None, kms64, bf64, var64
+    else:
+        sps32, dt32, A32, kms32, bf32, var32 = timed_run(inp, "f32", a.warmup, a.steps)
+    if a.prec == "both":
+        rel = float(np.linalg.norm((A32 - A64).ravel()) / np.linalg.norm(A64.ravel()))
+        log(f"[bench] fp32 vs fp64 rel L2 after {a.warmup + a.steps} steps: {rel:.3e}")
+        del A64
+    del A32
+    dtype = "f64" if a.prec == "f64" else "f32"
 
-    # dominant kernel of the FP32 path: the largest share of step time
-    shares = [kms32[i] * (1 if i == 0 else 1) for i in range(3)]
-    dom = int(np.argmax(shares))
+    # dominant kernel of the measured path: the largest share of step time
+    dom = int(np.argmax(kms32))
     per_launch_bytes = 2 * bf32           # read + write of the field
     achieved = per_launch_bytes / (kms32[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("f32"),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(dtype),
             "kernel": KERNELS[dom], "kernel_ms": round(kms32[dom], 4),
             "all_kernels_ms": {KERNELS[i]: round(kms32[i], 4) for i in range(3)},
             "algorithmic_bytes_per_launch": per_launch_bytes}
@@ -168,15 +177,16 @@ def main(argv=None):
     out = {
         "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * dt32 / a.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (seeded Gaussian pulse + 1e-6 complex noise, argon 2 bar)",
         "config": {"workload": f"{wl.nx}x{wl.nx}x{wl.nt} split-step, 1 mJ 300 fs, Ar 2 bar, ADK + SST on, "
                                f"dz = 1 mm (500 steps/pass), curved mirror R = 0.3 m",
-                   "precision": "fp32 complex, fp64 propagator phase / ADK exponent / density scan",
-                   "plain_fp64_steps_per_s": round(sps64, 3),
-                   "fp32_over_fp64": round(sps32 / sps64, 3),
+                   "precision": ("fp64 complex" if dtype == "f64" else
+                                 "fp32 complex, fp64 propagator phase / ADK exponent / density scan"),
+                   "plain_fp64_steps_per_s": round(sps64, 3) if sps64 else None,
+                   "fp32_over_fp64": round(sps32 / sps64, 3) if a.prec == "both" else None,
                    "fp32_vs_fp64_rel_l2": rel,
-                   "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)},
+                   "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)} if kms64 else None,
                    "autotuned_variants": {"f32": var32, "f64": var64}},
         "roofline": roof,
         "cpu_baseline": cpu,

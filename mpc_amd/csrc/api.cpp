@@ -97,6 +97,7 @@ static int build_pk(mpc_ctx *c) {
     k.I1e = I1 * EL0;
     k.Kon = p.deltat / T0;
     k.deltat = p.deltat;
+    k.dt_I1e = p.deltat / k.I1e;
     k.rho_nt = p.rho_nt;
     k.zf = 2.0 * ZW0 / NC;
     k.cs = k.zf * k.hbar_w0;

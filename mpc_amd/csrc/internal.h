@@ -40,6 +40,7 @@ struct PlasmaK {
     double I1e;          /* I1[eV] * EL0 */
     double Kon;          /* deltat / T0 */
     double deltat;
+    double dt_I1e;       /* deltat / I1e: the avalanche rate's factor */
     double rho_nt;
 };
 

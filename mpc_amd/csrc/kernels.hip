@@ -78,13 +78,13 @@ __device__ __forceinline__ double adk_rate(double ka, double ke, double L, doubl
     return (w != w) ? 0.0 : w;
 }
 
-/* field scale s = sqrt(2 ZW0 I / NC) from |A|^2, with I floored at 1e-30
- * (plasma.c:135-138, :147) */
-__device__ __forceinline__ double plasma_s(double a2, const PlasmaK &k, double *Iout) {
+/* 1/s, s = sqrt(2 ZW0 I / NC) the field scale from |A|^2, with I floored at
+ * 1e-30 (plasma.c:135-138, :147) */
+__device__ __forceinline__ double plasma_invs(double a2, const PlasmaK &k, double *Iout) {
     double I = a2 * k.hbar_w0;
     if (I == 0.0) I = 1e-30;
     *Iout = I;
-    return sqrt(k.zf * I);
+    return rsqrt(k.zf * I);
 }
 /* collision time and inverse-bremsstrahlung cross-section from 1/s
  * (plasma.c:207-210, quiver velocity at w0) */
@@ -105,13 +105,15 @@ template <class G> __device__ __forceinline__ int padp(int i) { return i + i / G
 
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
- * f_k(R) = M[k] R + C[k]; the result R_k = f_k(R_{k+1}) with R_NT = 0 (or,
- * with U, 1 - exp(-R_k)) is written over M[k].  Thread tid owns the consecutive segment
- * [tid*E, tid*E + E); segment maps are composed serially, combined across
- * lanes with a shuffle-based suffix scan and across wavefronts through LDS.
+ * f_k(R) = m_k R + c_k, given by coef(k, m_k, c_k); the result
+ * R_k = f_k(R_{k+1}), R_NT = 0, is handed to out(k, R_k).  Thread tid owns the
+ * consecutive segment [tid*E, tid*E + E): segment maps are composed serially,
+ * combined across lanes with a shuffle-based suffix scan and across wavefronts
+ * through LDS (scr), then each thread re-walks its segment.  coef must give
+ * the same values on both walks; out may overwrite what coef read at k.
  */
-template <class G, bool U = false>
-__device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const double *C, double *scr) {
+template <class G, class Coef, class Out>
+__device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef coef, Out out) {
     constexpr int E = G::E, TPL = G::NT / E;
     constexpr int W = TPL < 64 ? TPL : 64;
     constexpr int NW = (TPL + 63) / 64;
@@ -119,8 +121,8 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const dou
     double gm = 1.0, gc = 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = padp<G>(lo + i);
-        const double mk = M[k], ck = C[k];
+        double mk, ck;
+        coef(lo + i, mk, ck);
         gc = mk * gc + ck;
         gm = mk * gm;
     }
@@ -149,9 +151,10 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *M, const dou
     double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
-        const int k = padp<G>(lo + i);
-        R = M[k] * R + C[k];
-        M[k] = U ? one_minus_exp_neg(R) : R;   /* U: store u = 1 - exp(-R) instead of R */
+        double mk, ck;
+        coef(lo + i, mk, ck);
+        R = mk * R + ck;
+        out(lo + i, R);
     }
     __syncthreads();
 }
@@ -169,7 +172,11 @@ template <class G> struct PlasmaLds {
  *   u_k     = 1 - exp(-R_k)      (evaluated as -expm1(-R_k))
  *   ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1},            ions2_NT = 0
  *   ions1_k = u_k - ions2_k ;  F_k = (ions1_k + 2 ions2_k) sigma_k
- * Element k = tid + TPL*m of the line is held by thread tid in slot m.
+ * Element k = tid + TPL*m of the line is held by thread tid in slot m.  Two
+ * padded LDS arrays: U holds m_k = 1 - a_k, then u_k; X holds the first
+ * scan's c_k, then c_k of the second species, then ions2_k.  The second scan
+ * forms its coefficients from U and X as it walks, so no per-element state
+ * beyond 1/s stays in registers across the scans.
  * sink(m, F, tau) receives the result per element.
  */
 template <class T, class G, class Sink>
@@ -178,7 +185,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     using L = PlasmaLds<G>;
     constexpr int NT = G::NT;
     constexpr int E = L::E, TPL = L::TPL;
-    double *Lm = lds, *Lc = lds + L::ARR, *scr = lds + 2 * L::ARR;
+    double *U = lds, *X = lds + L::ARR, *scr = lds + 2 * L::ARR;
     int anyc = 0;
     double invs[E];
 #pragma unroll
@@ -186,44 +193,41 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         const int k = tid + TPL * m;
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
         double I;
-        const double sv = plasma_s(a2, pk, &I);
-        invs[m] = 1.0 / sv;
+        invs[m] = plasma_invs(a2, pk, &I);
         double tc;
         const double sig = plasma_sigma(invs[m], pk, &tc);
         const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, invs[m]) * pk.Kon;
-        const double a = sig * I / pk.I1e * pk.deltat;   /* avalanche, SI units */
-        Lm[padp<G>(k)] = 1.0 - a;
-        Lc[padp<G>(k)] = b * (1.0 - a) + a;
+        const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
+        U[padp<G>(k)] = 1.0 - a;
+        X[padp<G>(k)] = b * (1.0 - a) + a;
         anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= 760.0);
     }
     __syncthreads();
-    affine_suffix_scan<G, true>(tid, Lm, Lc, scr);    /* Lm <- u = 1 - exp(-R) */
+    /* U <- u = 1 - exp(-R) */
+    affine_suffix_scan<G>(tid, scr,
+        [&](int k, double &mk, double &ck) { mk = U[padp<G>(k)]; ck = X[padp<G>(k)]; },
+        [&](int k, double R) { U[padp<G>(k)] = one_minus_exp_neg(R); });
     const int block_anyc = __syncthreads_or(anyc);
-    double u[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) u[m] = Lm[padp<G>(tid + TPL * m)];
     if (block_anyc) {
-        /* second scan: ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1} */
-        double cp[E];
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int k = tid + TPL * m;
-            const double un = (k + 1 < NT) ? Lm[padp<G>(k + 1)] : 0.0;
-            const double c = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, invs[m]) * pk.Kon;
-            Lc[padp<G>(k)] = 1.0 - c;     /* M' */
-            cp[m] = c * un;               /* C' */
-        }
+        for (int m = 0; m < E; ++m)
+            X[padp<G>(tid + TPL * m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, invs[m]) * pk.Kon;
         __syncthreads();
-#pragma unroll
-        for (int m = 0; m < E; ++m) Lm[padp<G>(tid + TPL * m)] = cp[m];
-        __syncthreads();
-        affine_suffix_scan<G>(tid, Lc, Lm, scr);      /* Lc <- ions2 */
+        /* X <- ions2: m_k = 1 - c_k, c'_k = c_k u_{k+1} */
+        affine_suffix_scan<G>(tid, scr,
+            [&](int k, double &mk, double &ck) {
+                const double c = X[padp<G>(k)];
+                const double un = (k + 1 < NT) ? U[padp<G>(k + 1)] : 0.0;
+                mk = 1.0 - c;
+                ck = c * un;
+            },
+            [&](int k, double R) { X[padp<G>(k)] = R; });
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
-        const double i2 = block_anyc ? Lc[padp<G>(k)] : 0.0;
-        const double i1 = u[m] - i2;
+        const double i2 = block_anyc ? X[padp<G>(k)] : 0.0;
+        const double i1 = U[padp<G>(k)] - i2;
         const double fr = i1 + 2.0 * i2;
         double tc;
         const double sig = plasma_sigma(invs[m], pk, &tc);
