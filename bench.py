@@ -39,11 +39,12 @@ def log(*a):
 
 def variant_sweep(inp, prec, steps=3):
     """per-variant kernel times (ms): the numbers the autotuner chooses from"""
-    from mpc_amd import Propagator
+    from mpc_amd import Propagator, variant_counts
     out = {}
     p = Propagator(inp, prec)
-    for tv in range(2):
-        for av in range(4):
+    ntv, nav = variant_counts()
+    for tv in range(ntv):
+        for av in range(nav):
             p.upload(inp.A)
             p.calibrate()
             p.set_variants(tv, av)
@@ -62,7 +63,7 @@ def timed_run(inp, prec, warmup, steps, with_kernel_timing=True):
     p.upload(inp.A)
     p.calibrate()
     variants = p.autotune()
-    log(f"[bench] {prec}: autotuned variants (t-line, axis) = {variants}")
+    log(f"[bench] {prec}: autotuned variants (t-line, x-line, y-line) = {variants}")
     t0 = time.perf_counter()
     p.run(0, warmup)
     log(f"[bench] {prec}: warmup {warmup} steps {time.perf_counter() - t0:.2f}s")

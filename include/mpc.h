@@ -106,8 +106,14 @@ int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int ma
  * compute the same transform; the choice only changes speed.  The
  * MPC_TLINE_VARIANT / MPC_AXIS_VARIANT environment variables pin them. */
 int  mpc_autotune(mpc_ctx *ctx);
-int  mpc_get_variants(mpc_ctx *ctx, int *tline_variant, int *axis_variant);
-int  mpc_set_variants(mpc_ctx *ctx, int tline_variant, int axis_variant);
+int  mpc_get_variants(mpc_ctx *ctx, int *tline_variant, int *axis_variant);   /* axis: the x-line's */
+int  mpc_set_variants(mpc_ctx *ctx, int tline_variant, int axis_variant);    /* same variant for x and y */
+/* per-axis form: the x- and y-line sweeps are tuned separately (the y lines'
+ * stride is Nx times the x lines', so their best tile order differs) */
+int  mpc_get_variants_xy(mpc_ctx *ctx, int *tline_variant, int *x_variant, int *y_variant);
+int  mpc_set_variants_xy(mpc_ctx *ctx, int tline_variant, int x_variant, int y_variant);
+/* number of variants of each kind: t-line, and x/y-line (plan + 4 * tile order) */
+int  mpc_variant_counts(int *tline_variants, int *axis_variants);
 
 /* Timing of the most recent mpc_run: average device time per launch of each
  * sweep kernel, measured with HIP events on the library's stream.

@@ -91,6 +91,13 @@ def _bind_arrays(inp: Inputs) -> None:
     p.betas, p.wr, p.w, p.r = _dp(inp.betas), _dp(inp.wr), _dp(inp.w), _dp(inp.r)
 
 
+def variant_counts() -> tuple:
+    """(number of t-line kernel variants, number of x/y-line kernel variants)"""
+    tv, av = C.c_int(), C.c_int()
+    lib().mpc_variant_counts(C.byref(tv), C.byref(av))
+    return tv.value, av.value
+
+
 class Propagator:
     """A device-resident field and its run context (mpc_ctx)."""
 
@@ -166,12 +173,13 @@ class Propagator:
         return self.variants()
 
     def variants(self) -> tuple:
-        tv, av = C.c_int(), C.c_int()
-        check(lib().mpc_get_variants(self.ctx, C.byref(tv), C.byref(av)), "mpc_get_variants")
-        return tv.value, av.value
+        """(t-line, x-line, y-line) kernel variants in use"""
+        tv, xv, yv = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mpc_get_variants_xy(self.ctx, C.byref(tv), C.byref(xv), C.byref(yv)), "mpc_get_variants_xy")
+        return tv.value, xv.value, yv.value
 
-    def set_variants(self, tline: int, axis: int) -> None:
-        check(lib().mpc_set_variants(self.ctx, tline, axis), "mpc_set_variants")
+    def set_variants(self, tline: int, x: int, y: int | None = None) -> None:
+        check(lib().mpc_set_variants_xy(self.ctx, tline, x, x if y is None else y), "mpc_set_variants_xy")
 
     def set_timing(self, on: bool) -> None:
         check(lib().mpc_set_timing(self.ctx, 1 if on else 0), "mpc_set_timing")

@@ -94,6 +94,9 @@ def lib() -> C.CDLL:
         "mpc_autotune": (C.c_int, [vp]),
         "mpc_get_variants": (C.c_int, [vp, iptr, iptr]),
         "mpc_set_variants": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpc_get_variants_xy": (C.c_int, [vp, iptr, iptr, iptr]),
+        "mpc_set_variants_xy": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mpc_variant_counts": (C.c_int, [iptr, iptr]),
         "mpc_mkinput": (C.c_int, [C.POINTER(MpcInputSpec), C.POINTER(MpcParams), dptr, dptr, dptr, dptr, vp]),
         "mpc_write_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),
         "mpc_read_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),

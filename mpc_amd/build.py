@@ -31,7 +31,10 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wno-unused-result"]
 
-HIP_SOURCES = ["kernels.hip", "api.cpp"]
+HIP_SOURCES = ["tline.hip", "kernels.hip", "api.cpp"]
+# per-file flags: the persistent t-line kernel keeps machine-level LICM from
+# hoisting exp/log polynomial constants out of its line loop into registers
+EXTRA_FLAGS = {"tline.hip": ["-mllvm", "-disable-machine-licm"]}
 C_SOURCES = ["host.c"]
 
 
@@ -50,7 +53,7 @@ def build_library(verbose: bool = False) -> str:
         o = os.path.join(OUT, os.path.splitext(s)[0] + ".o")
         src = os.path.join(CSRC, s)
         lang = ["-x", "hip"] if s.endswith(".cpp") else []
-        jobs.append([HIPCC, *HIP_FLAGS, *lang, "-c", src, "-o", o])
+        jobs.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(s, []), *lang, "-c", src, "-o", o])
         objs.append(o)
     for s in C_SOURCES:
         o = os.path.join(OUT, os.path.splitext(s)[0] + ".o")

@@ -207,7 +207,7 @@ extern "C" int mpc_create(const mpc_params *p, int precision, mpc_ctx **out) {
     if (!rc) rc = ctx_set_params(c, p);
     if (rc) { mpc_destroy(c); return rc; }
     if (const char *e = getenv("MPC_TLINE_VARIANT")) c->tvar = std::max(0, std::min(tline_variants() - 1, atoi(e)));
-    if (const char *e = getenv("MPC_AXIS_VARIANT")) c->avar = std::max(0, std::min(axis_variants() - 1, atoi(e)));
+    if (const char *e = getenv("MPC_AXIS_VARIANT")) c->avar = c->avar_y = std::max(0, std::min(axis_variants() - 1, atoi(e)));
     *out = c;
     return 0;
 }
@@ -311,13 +311,15 @@ static void resolve_timing(mpc_ctx *c) {
     c->evn = 0;
 }
 
-template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror) {
+template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror,
+                                         const cx<T> *Pt = nullptr) {
     AArgs<T> a{};
     a.field = (cx<T> *)c->field;
     a.tw = (const cx<T> *)c->tw_x;
     a.D = D;
     a.nt = c->nt;
     a.M = nullptr;
+    a.Pt = Pt;
     if (axis == 0) {        /* x-lines: tile at fixed y */
         a.s_line = c->nt;
         a.s_outer = (long)c->nx * c->nt;
@@ -328,17 +330,20 @@ template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, b
         if (mirror) { a.M = (const cx<T> *)c->mirror; a.m_line = c->nx; a.m_outer = 1; }
     }
     tic(c, 1 + axis);
-    const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx, c->avar, c->stream);
+    const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx,
+                                  axis == 0 ? c->avar : c->avar_y, c->stream);
     toc(c, 1 + axis);
     return rc;
 }
 
-template <class T> static int run_xy(mpc_ctx *c, double coef, bool mirror) {
-    if (coef == 0.0 && !mirror) return 0;
+/* x then y sweep with the merged diffraction factor; Pt (t-spectral field
+ * only) is a per-t factor the y sweep applies before its store */
+template <class T> static int run_xy(mpc_ctx *c, double coef, bool mirror, const cx<T> *Pt = nullptr) {
+    if (coef == 0.0 && !mirror && !Pt) return 0;
     const cx<T> *D = get_table<T>(c, 'x', coef);
     if (!D) return MPC_ENOMEM;
     RCHK(sweep_axis<T>(c, 0, D, false));
-    return sweep_axis<T>(c, 1, D, mirror);
+    return sweep_axis<T>(c, 1, D, mirror, Pt);
 }
 
 template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q, bool in_freq, bool do_nl,
@@ -364,7 +369,9 @@ template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q
     const bool plasma = do_nl && c->p.plasm == 1;
     a.rho_block = plasma ? c->rho_block : nullptr;
     tic(c, 0);
-    const int rc = launch_tline<T>(a, c->nt, plasma, c->tvar, c->stream);
+    int nb = 0;
+    const int rc = launch_tline<T>(a, c->nt, plasma, c->tvar, c->stream, &nb);
+    if (!rc && plasma) c->rho_nvalid = nb;
     toc(c, 0);
     return rc;
 }
@@ -385,12 +392,16 @@ template <class T> static int run_fixed(mpc_ctx *c, int step0, int nsteps) {
         const int s = pass_sign(c, z);
         ct += dz / 2;
         cxy += s * dz / 2;
-        RCHK(run_xy<T>(c, cxy, false));
+        const cx<T> *P = get_table<T>(c, 't', ct);
+        if (!P) return MPC_ENOMEM;
+        /* on a t-spectral field the dispersion factor rides on the y sweep's
+         * store (the linear factors commute); the t-line kernel then skips it */
+        const bool p_in_y = freq && cxy != 0.0;
+        RCHK(run_xy<T>(c, cxy, false, p_in_y ? P : nullptr));
+        if (p_in_y) P = nullptr;
         cxy = 0.0;
         const bool last = (n == step0 + nsteps - 1);
         const bool mirror_after = ((n + 1) % S == 0) && ((n + 1) / S < npass);
-        const cx<T> *P = get_table<T>(c, 't', ct);
-        if (!P) return MPC_ENOMEM;
         if (last) {
             const cx<T> *Q = get_table<T>(c, 't', dz / 2);
             if (!Q) return MPC_ENOMEM;
@@ -562,7 +573,7 @@ extern "C" int mpc_diagnostics(mpc_ctx *c, mpc_diag *d) {
     if (d->BP2) for (int j = 0; j < c->ny; j++) { double s = 0; for (int i = 0; i < c->nx; i++) s += lp[(size_t)j * c->nx + i]; d->BP2[j] = s; }
     d->rho_peak = 0.0;
     if (c->p.plasm == 1) {
-        rb.resize((size_t)c->rho_nblocks);
+        rb.resize((size_t)std::min<long>(c->rho_nvalid, c->rho_nblocks));
         HIPCHK(hipMemcpy(rb.data(), c->rho_block, sizeof(double) * rb.size(), hipMemcpyDeviceToHost));
         double mx = 0.0;
         for (double v : rb) mx = std::max(mx, v);
@@ -674,7 +685,7 @@ template <class T> static int autotune(mpc_ctx *c) {
     if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
     HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, c->stream));
     void *keep = c->field;
-    const int keep_t = c->tvar, keep_a = c->avar, keep_timing = c->timing;
+    const int keep_t = c->tvar, keep_a = c->avar, keep_y = c->avar_y, keep_timing = c->timing;
     c->field = c->aux1;
     c->timing = 0;
     hipEvent_t e0, e1;
@@ -698,13 +709,14 @@ template <class T> static int autotune(mpc_ctx *c) {
     const cx<T> *D = get_table<T>(c, 'x', dz);
     const cx<T> *P = get_table<T>(c, 't', dz);
     int rc = (!D || !P) ? MPC_ENOMEM : 0;
-    float bestA = 1e30f, bestT = 1e30f;
-    int va = keep_a, vt = keep_t;
+    float bestX = 1e30f, bestY = 1e30f, bestT = 1e30f;
+    int vx = keep_a, vy = keep_y, vt = keep_t;
     for (int v = 0; !rc && v < axis_variants(); v++) {
-        c->avar = v;
-        const float ms = time3([&] { return sweep_axis<T>(c, 0, D, false); }) +
-                         time3([&] { return sweep_axis<T>(c, 1, D, false); });
-        if (ms < bestA) { bestA = ms; va = v; }
+        c->avar = c->avar_y = v;
+        const float mx = time3([&] { return sweep_axis<T>(c, 0, D, false); });
+        const float my = time3([&] { return sweep_axis<T>(c, 1, D, false); });
+        if (mx < bestX) { bestX = mx; vx = v; }
+        if (my < bestY) { bestY = my; vy = v; }
     }
     for (int v = 0; !rc && v < tline_variants(); v++) {
         c->tvar = v;
@@ -715,7 +727,8 @@ template <class T> static int autotune(mpc_ctx *c) {
     (void)hipEventDestroy(e1);
     c->field = keep;
     c->timing = keep_timing;
-    c->avar = rc ? keep_a : va;
+    c->avar = rc ? keep_a : vx;
+    c->avar_y = rc ? keep_y : vy;
     c->tvar = rc ? keep_t : vt;
     if (rc) return rc;
     c->tuned = 1;
@@ -733,10 +746,27 @@ extern "C" int mpc_get_variants(mpc_ctx *c, int *tv, int *av) {
     if (av) *av = c->avar;
     return 0;
 }
+extern "C" int mpc_variant_counts(int *tv, int *av) {
+    if (tv) *tv = tline_variants();
+    if (av) *av = axis_variants();
+    return 0;
+}
 extern "C" int mpc_set_variants(mpc_ctx *c, int tv, int av) {
-    if (!c || tv < 0 || tv >= tline_variants() || av < 0 || av >= axis_variants()) return MPC_EINVAL;
+    return mpc_set_variants_xy(c, tv, av, av);
+}
+extern "C" int mpc_get_variants_xy(mpc_ctx *c, int *tv, int *xv, int *yv) {
+    if (!c) return MPC_EINVAL;
+    if (tv) *tv = c->tvar;
+    if (xv) *xv = c->avar;
+    if (yv) *yv = c->avar_y;
+    return 0;
+}
+extern "C" int mpc_set_variants_xy(mpc_ctx *c, int tv, int xv, int yv) {
+    const int na = axis_variants();
+    if (!c || tv < 0 || tv >= tline_variants() || xv < 0 || xv >= na || yv < 0 || yv >= na) return MPC_EINVAL;
     c->tvar = tv;
-    c->avar = av;
+    c->avar = xv;
+    c->avar_y = yv;
     return 0;
 }
 
@@ -750,7 +780,7 @@ struct mpc_plan_s {
         mpc_ctx *ctx;
         std::string wisdom;   /* "<cpath><Nx>": the kernel-variant cache */
         int readin;
-        int tvar, avar;       /* from the cache, or -1 until tuned */
+        int tvar, avar, avar_y; /* from the cache, or -1 until tuned */
     } *sh;
     int axis, dir;
 };
@@ -783,12 +813,13 @@ extern "C" void createPlans(int readin, int *dims, const char *cpath, fftw_plan 
     /* the autotuner cache, read like FFTW wisdom (propagation.c:540-552);
      * it is written after the first tuning (:609-611) */
     sh->readin = readin;
-    sh->tvar = sh->avar = -1;
+    sh->tvar = sh->avar = sh->avar_y = -1;
     if (readin && !sh->wisdom.empty()) {
         FILE *f = fopen(sh->wisdom.c_str(), "r");
         if (f) {
-            int t = -1, a = -1;
-            if (fscanf(f, "mpc-kernel-variants tline=%d axis=%d", &t, &a) == 2) { sh->tvar = t; sh->avar = a; }
+            int t = -1, a = -1, y = -1;
+            const int nr = fscanf(f, "mpc-kernel-variants tline=%d axis=%d yaxis=%d", &t, &a, &y);
+            if (nr >= 2) { sh->tvar = t; sh->avar = a; sh->avar_y = nr == 3 ? y : a; }
             else printf("Importing wisdom failed, creating new one to save.\n");
             fclose(f);
         }
@@ -799,15 +830,16 @@ extern "C" void createPlans(int readin, int *dims, const char *cpath, fftw_plan 
 static int legacy_tune(mpc_plan_s::Shared *sh) {
     mpc_ctx *c = sh->ctx;
     if (c->tuned) return 0;
-    if (sh->tvar >= 0 && mpc_set_variants(c, sh->tvar, sh->avar) == 0) { c->tuned = 1; return 0; }
+    if (sh->tvar >= 0 && mpc_set_variants_xy(c, sh->tvar, sh->avar, sh->avar_y) == 0) { c->tuned = 1; return 0; }
     RCHK(mpc_autotune(c));
     sh->tvar = c->tvar;
     sh->avar = c->avar;
+    sh->avar_y = c->avar_y;
     if (!sh->wisdom.empty()) {
         FILE *f = fopen(sh->wisdom.c_str(), "w");
         if (f) {
-            fprintf(f, "mpc-kernel-variants tline=%d axis=%d nx=%d ny=%d nt=%d\n", c->tvar, c->avar, sh->dims[0],
-                    sh->dims[1], sh->dims[2]);
+            fprintf(f, "mpc-kernel-variants tline=%d axis=%d yaxis=%d nx=%d ny=%d nt=%d\n", c->tvar, c->avar,
+                    c->avar_y, sh->dims[0], sh->dims[1], sh->dims[2]);
             fclose(f);
         }
     }

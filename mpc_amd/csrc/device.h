@@ -1,0 +1,313 @@
+/*
+ * device.h -- device building blocks shared by the kernel translation units:
+ * the line-FFT driver over fft_core.h's stages, t-line geometry, the FP64
+ * plasma model (ADK rate, avalanche, reverse-t affine scan), and the
+ * nonlinear phase's sin/cos and exp.
+ */
+#ifndef MPC_DEVICE_H
+#define MPC_DEVICE_H
+
+#include "internal.h"
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+
+namespace mpc {
+
+/* ---------------------------------------------------------------------- */
+/* FFT driver: unrolls Plan<N>::R into stages with LDS exchange between    */
+/* ---------------------------------------------------------------------- */
+template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
+struct DevFft {
+    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
+                                               const cx<T> *tw) {
+        constexpr bool last = sizeof...(Rest) == 0;
+        fft_stage<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds, ls, tw);
+        if constexpr (!last) {
+            __syncthreads();
+            fft_gather<T, N, E, PAD>(v, tid, lds, ls);
+            __syncthreads();
+            DevFft<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
+        }
+    }
+};
+template <class T, int N, int E, int DIR, int PAD, class Rad> struct DevFftP;
+template <class T, int N, int E, int DIR, int PAD, int... Rs> struct DevFftP<T, N, E, DIR, PAD, Radices<Rs...>> {
+    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+        DevFft<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, tw);
+    }
+};
+/* line FFT with plan PL (PL::E elements per thread, radix list PL::R) */
+template <class T, int N, class PL, int DIR, int PAD = 0>
+__device__ __forceinline__ void line_fft(cx<T> (&v)[PL::E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+    DevFftP<T, N, PL::E, DIR, PAD, typename PL::R>::run(v, tid, lds, ls, tw);
+}
+
+/* t-line geometry: plan, LDS padding and lines per 256-thread block.  The
+ * LDS pad slot every E elements matches the stride of the first Stockham
+ * stage's writes. */
+template <int NT_, class PL_ = TPlan<NT_>> struct TGeo {
+    static constexpr int NT = NT_;
+    using PL = PL_;
+    static constexpr int E = PL::E, TPL = NT / E;
+    static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
+    static constexpr int THREADS = LPB * TPL;
+    static constexpr int PAD = E;
+    static constexpr int FFT_ELEMS = NT + NT / PAD;
+};
+
+/* ---------------------------------------------------------------------- */
+/* plasma model pieces (FP64 in both precisions)                           */
+/* ---------------------------------------------------------------------- */
+/* ADK rate Kon_a (Kon_b/Ea)^Kon_c exp(-Kon_b/(3 Ea)) (plasma.c:55-56) in log
+ * form with e = Kon_b/(3 Ea) = ke * (1/s) and log(Kon_b/Ea) = L + log(1/s);
+ * exactly 0 where the exponential underflows in double precision. */
+__device__ __forceinline__ double adk_rate(double ka, double ke, double L, double kc, double invs) {
+    const double e = ke * invs;
+    if (ka == 0.0 || e > 760.0) return 0.0;
+    const double w = ka * exp(kc * (L + log(invs)) - e);
+    return (w != w) ? 0.0 : w;
+}
+
+/* 1/s, s = sqrt(2 ZW0 I / NC) the field scale from |A|^2, with I floored at
+ * 1e-30 (plasma.c:135-138, :147) */
+__device__ __forceinline__ double plasma_invs(double a2, const PlasmaK &k, double *Iout) {
+    double I = a2 * k.hbar_w0;
+    if (I == 0.0) I = 1e-30;
+    *Iout = I;
+    return rsqrt(k.zf * I);
+}
+/* collision time and inverse-bremsstrahlung cross-section from 1/s
+ * (plasma.c:207-210, quiver velocity at w0) */
+__device__ __forceinline__ double plasma_sigma(double invs, const PlasmaK &k, double *tau) {
+    const double tc = k.kt * invs;
+    *tau = tc;
+    return k.K4 * tc / (1.0 + k.K5 * (tc * tc));
+}
+/* -expm1(-R): a 5-term series below 1e-4 (truncation < 1e-20 relative) */
+__device__ __forceinline__ double one_minus_exp_neg(double R) {
+    if (R < 1e-4) return R * (1.0 + R * (-0.5 + R * (1.0 / 6.0 + R * (-1.0 / 24.0 + R * (1.0 / 120.0)))));
+    return -expm1(-R);
+}
+
+/* plasma arrays: one pad slot per E-element segment, so the per-thread
+ * consecutive segments of the scan start in different LDS banks */
+template <class G> __device__ __forceinline__ int padp(int i) { return i + i / G::E; }
+
+/*
+ * Reverse-t affine scan over one line.  Element k carries the map
+ * f_k(R) = m_k R + c_k, given by coef(k, m_k, c_k); the result
+ * R_k = f_k(R_{k+1}), R_NT = 0, is handed to out(k, R_k).  Thread tid owns the
+ * consecutive segment [tid*E, tid*E + E): segment maps are composed serially,
+ * combined across lanes with a shuffle-based suffix scan and across wavefronts
+ * through LDS (scr), then each thread re-walks its segment.  coef must give
+ * the same values on both walks; out may overwrite what coef read at k.
+ */
+template <class G, class Coef, class Out>
+__device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef coef, Out out) {
+    constexpr int E = G::E, TPL = G::NT / E;
+    constexpr int W = TPL < 64 ? TPL : 64;
+    constexpr int NW = (TPL + 63) / 64;
+    const int lo = tid * E;
+    double gm = 1.0, gc = 0.0;
+#pragma unroll
+    for (int i = E - 1; i >= 0; --i) {
+        double mk, ck;
+        coef(lo + i, mk, ck);
+        gc = mk * gc + ck;
+        gm = mk * gm;
+    }
+    const int lane = tid % W;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        const double om = __shfl_down(gm, d, W);
+        const double oc = __shfl_down(gc, d, W);
+        if (lane + d < W) {
+            gc = gm * oc + gc;
+            gm = gm * om;
+        }
+    }
+    if constexpr (NW > 1) {
+        const int wv = tid / 64;
+        if (lane == 0) { scr[2 * wv] = gm; scr[2 * wv + 1] = gc; }
+        __syncthreads();
+        for (int w2 = wv + 1; w2 < NW; ++w2) {
+            const double om = scr[2 * w2], oc = scr[2 * w2 + 1];
+            gc = gm * oc + gc;
+            gm = gm * om;
+        }
+    }
+    scr[2 * NW + tid] = gc;
+    __syncthreads();
+    double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
+#pragma unroll
+    for (int i = E - 1; i >= 0; --i) {
+        double mk, ck;
+        coef(lo + i, mk, ck);
+        R = mk * R + ck;
+        out(lo + i, R);
+    }
+    __syncthreads();
+}
+
+template <class G> struct PlasmaLds {
+    static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
+    static constexpr int ARR = NT + NT / E;            /* padded doubles per array */
+    static constexpr int SCR = 2 * NW + TPL;
+    static constexpr int BYTES = (2 * ARR + SCR) * 8;
+};
+
+/*
+ * Density of one line (plasma.c:218-241, the consistent two-species form):
+ *   R_k     = (R_{k+1} + b_k)(1 - a_k) + a_k,                 R_NT = 0
+ *   u_k     = 1 - exp(-R_k)      (evaluated as -expm1(-R_k))
+ *   ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1},            ions2_NT = 0
+ *   ions1_k = u_k - ions2_k ;  F_k = (ions1_k + 2 ions2_k) sigma_k
+ * Element k = tid + TPL*m of the line is held by thread tid in slot m.  Two
+ * padded LDS arrays: U holds m_k = 1 - a_k, then u_k; X holds the first
+ * scan's c_k, then c_k of the second species, then ions2_k.  The second scan
+ * forms its coefficients from U and X as it walks, so no per-element state
+ * beyond 1/s stays in registers across the scans.
+ * sink(m, F, tau) receives the result per element.
+ */
+template <class T, class G, class Sink>
+__device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, double *lds,
+                                            const PlasmaK &pk, double &rho_max, Sink sink) {
+    using L = PlasmaLds<G>;
+    constexpr int NT = G::NT;
+    constexpr int E = L::E, TPL = L::TPL;
+    double *U = lds, *X = lds + L::ARR, *scr = lds + 2 * L::ARR;
+    int anyc = 0;
+    double invs[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = tid + TPL * m;
+        const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
+        double I;
+        invs[m] = plasma_invs(a2, pk, &I);
+        double tc;
+        const double sig = plasma_sigma(invs[m], pk, &tc);
+        const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, invs[m]) * pk.Kon;
+        const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
+        U[padp<G>(k)] = 1.0 - a;
+        X[padp<G>(k)] = b * (1.0 - a) + a;
+        anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= 760.0);
+    }
+    __syncthreads();
+    /* U <- u = 1 - exp(-R) */
+    affine_suffix_scan<G>(tid, scr,
+        [&](int k, double &mk, double &ck) { mk = U[padp<G>(k)]; ck = X[padp<G>(k)]; },
+        [&](int k, double R) { U[padp<G>(k)] = one_minus_exp_neg(R); });
+    const int block_anyc = __syncthreads_or(anyc);
+    if (block_anyc) {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+            X[padp<G>(tid + TPL * m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, invs[m]) * pk.Kon;
+        __syncthreads();
+        /* X <- ions2: m_k = 1 - c_k, c'_k = c_k u_{k+1} */
+        affine_suffix_scan<G>(tid, scr,
+            [&](int k, double &mk, double &ck) {
+                const double c = X[padp<G>(k)];
+                const double un = (k + 1 < NT) ? U[padp<G>(k + 1)] : 0.0;
+                mk = 1.0 - c;
+                ck = c * un;
+            },
+            [&](int k, double R) { X[padp<G>(k)] = R; });
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = tid + TPL * m;
+        const double i2 = block_anyc ? X[padp<G>(k)] : 0.0;
+        const double i1 = U[padp<G>(k)] - i2;
+        const double fr = i1 + 2.0 * i2;
+        double tc;
+        const double sig = plasma_sigma(invs[m], pk, &tc);
+        rho_max = fmax(rho_max, fr);
+        sink(m, fr * sig, tc);
+    }
+    __syncthreads();
+}
+
+/* ---------------------------------------------------------------------- */
+/* t-line kernel                                                           */
+/* ---------------------------------------------------------------------- */
+template <class T, class G, bool PLASMA> struct TCfg {
+    static constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB, THREADS = G::THREADS;
+    static constexpr int FFT_BYTES = G::FFT_ELEMS * (int)sizeof(cx<T>);
+    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<G>::BYTES : 0;
+    static constexpr int LINE_BYTES = ((FFT_BYTES > PL_BYTES ? FFT_BYTES : PL_BYTES) + 15) / 16 * 16;
+};
+
+/*
+ * sin/cos of the per-step nonlinear phase (small in practice).  Cody-Waite
+ * reduction by pi/2 with a three-part constant, then Taylor polynomials on
+ * [-pi/4, pi/4] (truncation below 1e-19 in FP64, 2e-9 in FP32) and the
+ * quadrant swap.  |x| > 1e5 takes the library routine instead.
+ */
+static __device__ __noinline__ void sincos_big(double x, double *s, double *c) { sincos(x, s, c); }
+static __device__ __noinline__ void sincos_bigf(float x, float *s, float *c) { sincosf(x, s, c); }
+
+template <class T> __device__ __forceinline__ void sincos_t(T x, T *s, T *c);
+template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) {
+    if (!(fabs(x) <= 1e5)) { sincos_big(x, s, c); return; }
+    const double k = rint(x * 0.63661977236758134308);          /* 2/pi */
+    double r = fma(-k, 1.57079632673412561417e+00, x);            /* pi/2 = h + m + l */
+    r = fma(-k, 6.07710050630396597660e-11, r);
+    r = fma(-k, 2.02226624879595063154e-21, r);
+    const double r2 = r * r;
+    /* sin r = r - r^3/3! + ... ; cos r = 1 - r^2/2! + ... (Horner in r^2) */
+    double sp = -1.0 / 121645100408832000.0;                      /* -1/19! */
+    sp = fma(sp, r2, 1.0 / 355687428096000.0);                    /* 1/17! */
+    sp = fma(sp, r2, -1.0 / 1307674368000.0);                     /* -1/15! */
+    sp = fma(sp, r2, 1.0 / 6227020800.0);                         /* 1/13! */
+    sp = fma(sp, r2, -1.0 / 39916800.0);                          /* -1/11! */
+    sp = fma(sp, r2, 1.0 / 362880.0);                             /* 1/9! */
+    sp = fma(sp, r2, -1.0 / 5040.0);                              /* -1/7! */
+    sp = fma(sp, r2, 1.0 / 120.0);                                /* 1/5! */
+    sp = fma(sp, r2, -1.0 / 6.0);                                 /* -1/3! */
+    const double sr = fma(sp * r2, r, r);
+    double cp = 1.0 / 6402373705728000.0;                         /* 1/18! */
+    cp = fma(cp, r2, -1.0 / 20922789888000.0);                    /* -1/16! */
+    cp = fma(cp, r2, 1.0 / 87178291200.0);                        /* 1/14! */
+    cp = fma(cp, r2, -1.0 / 479001600.0);                         /* -1/12! */
+    cp = fma(cp, r2, 1.0 / 3628800.0);                            /* 1/10! */
+    cp = fma(cp, r2, -1.0 / 40320.0);                             /* -1/8! */
+    cp = fma(cp, r2, 1.0 / 720.0);                                /* 1/6! */
+    cp = fma(cp, r2, -1.0 / 24.0);                                /* -1/4! */
+    cp = fma(cp, r2, 0.5);
+    const double cr = fma(-cp, r2, 1.0);
+    const int q = ((int)k) & 3;
+    const double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) {
+    if (!(fabsf(x) <= 1e5f)) { sincos_bigf(x, s, c); return; }
+    const float k = rintf(x * 0.636619772f);
+    float r = fmaf(-k, 1.5703125f, x);                            /* pi/2 in three parts */
+    r = fmaf(-k, 4.8351287841796875e-4f, r);
+    r = fmaf(-k, 3.13916473e-7f, r);
+    const float r2 = r * r;
+    float sp = 1.0f / 362880.0f;                                  /* Taylor to r^9 */
+    sp = fmaf(sp, r2, -1.0f / 5040.0f);
+    sp = fmaf(sp, r2, 1.0f / 120.0f);
+    sp = fmaf(sp, r2, -1.0f / 6.0f);
+    const float sr = fmaf(sp * r2, r, r);
+    float cp = -1.0f / 3628800.0f;                                /* Taylor to r^10 */
+    cp = fmaf(cp, r2, 1.0f / 40320.0f);
+    cp = fmaf(cp, r2, -1.0f / 720.0f);
+    cp = fmaf(cp, r2, 1.0f / 24.0f);
+    cp = fmaf(cp, r2, -0.5f);
+    const float cr = fmaf(cp, r2, 1.0f);
+    const int q = ((int)k) & 3;
+    const float ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+template <class T> __device__ __forceinline__ T exp_t(T x);
+template <> __device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
+
+} // namespace mpc
+
+#endif

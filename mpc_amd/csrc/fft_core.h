@@ -158,46 +158,83 @@ template <int PAD> MPC_HD int padi(int i) { return PAD ? i + i / PAD : i; }
 /*
  * Stage twiddles w[r] = W^(r*idx) (conjugated for the inverse), r < R, from
  * the table W^m = tw[m] = exp(-2 pi i m / N).  Only the power-of-two powers
- * are loaded; the rest are products of loaded entries (at most three
- * multiplications deep), which cuts the table loads of a radix-8 stage from
- * 7 to 3 and of a radix-16 stage from 15 to 4.  r*idx < N, so no wrap.
+ * are loaded (load_powers); the rest are products of loaded entries, at most
+ * three multiplications deep (expand_powers).  That cuts the table loads of a
+ * radix-8 stage from 7 to 3 and of a radix-16 stage from 15 to 4.  r*idx < N,
+ * so no wrap.  The split lets a kernel load the powers once and expand them
+ * for every transform it runs.
  */
+template <int R> constexpr int tw_npow() {
+    return R == 2 ? 1 : (R == 3 || R == 4) ? 2 : R == 8 ? 3 : R == 16 ? 4 : R - 1;
+}
+template <int R, class T>
+MPC_HD void load_powers(cx<T> (&p)[tw_npow<R>()], const cx<T> *tw, int idx) {
+    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) {
+        p[0] = tw[idx];
+        if constexpr (R > 2) p[1] = tw[2 * idx];
+        if constexpr (R >= 8) p[2] = tw[4 * idx];
+        if constexpr (R == 16) p[3] = tw[8 * idx];
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) p[r - 1] = tw[r * idx];
+    }
+}
 template <int R, int DIR, class T>
-MPC_HD void stage_twiddles(cx<T> (&w)[R], const cx<T> *tw, int idx) {
-    auto ld = [&](int m) {
-        cx<T> v = tw[m];
+MPC_HD void expand_powers(cx<T> (&w)[R], const cx<T> (&p)[tw_npow<R>()]) {
+    auto cj = [](cx<T> v) {
         if (DIR > 0) v.im = -v.im;
         return v;
     };
     w[0] = mk((T)1, (T)0);
-    if (R == 2) { w[1] = ld(idx); return; }
-    if (R == 3) { w[1] = ld(idx); w[2] = ld(2 * idx); return; }
-    w[1] = ld(idx);
-    w[2] = ld(2 * idx);
-    w[3] = cmul(w[1], w[2]);
-    if (R == 4) return;
-    if (R == 12) {
-        for (int r = 4; r < R; ++r) w[r] = ld(r * idx);
-        return;
-    }
-    w[4] = ld(4 * idx);
-    w[5] = cmul(w[1], w[4]);
-    w[6] = cmul(w[2], w[4]);
-    w[7] = cmul(w[3], w[4]);
-    if (R == 8) return;
-    w[8] = ld(8 * idx);
+    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) {
+        w[1] = cj(p[0]);
+        if constexpr (R > 2) {
+            w[2] = cj(p[1]);
+            if constexpr (R > 3) {
+                w[3] = cmul(w[1], w[2]);
+                if constexpr (R > 4) {
+                    w[4] = cj(p[2]);
+                    w[5] = cmul(w[1], w[4]);
+                    w[6] = cmul(w[2], w[4]);
+                    w[7] = cmul(w[3], w[4]);
+                    if constexpr (R > 8) {
+                        w[8] = cj(p[3]);
 #pragma unroll
-    for (int r = 9; r < R; ++r) w[r] = cmul(w[r - 8], w[8]);
+                        for (int r = 9; r < R; ++r) w[r] = cmul(w[r - 8], w[8]);
+                    }
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = cj(p[r - 1]);
+    }
 }
+template <int R, int DIR, class T>
+MPC_HD void stage_twiddles(cx<T> (&w)[R], const cx<T> *tw, int idx) {
+    cx<T> p[tw_npow<R>()];
+    load_powers<R>(p, tw, idx);
+    expand_powers<R, DIR>(w, p);
+}
+
+/* twiddle sources for fft_stage: tw(b, k, w) fills w[R] for butterfly b
+ * (of the thread's E/R) whose in-group index is k */
+template <class T, int N, int R, int Ns, int DIR> struct TwFromTable {
+    const cx<T> *tw;
+    MPC_HD void operator()(int, int k, cx<T> (&w)[R]) const { stage_twiddles<R, DIR>(w, tw, k * (N / (Ns * R))); }
+};
+template <class T, int R, int B, int DIR> struct TwFromPowers {
+    const cx<T> (*p)[tw_npow<R>()];   /* p[b] */
+    MPC_HD void operator()(int b, int, cx<T> (&w)[R]) const { expand_powers<R, DIR>(w, p[b]); }
+};
 
 /*
  * One Stockham stage.  Ns = product of the radices of earlier stages.
  * v: the thread's E registers (x[tid + TPL*m]); lds: the line's exchange
- * buffer, element idx at lds[padi<PAD>(idx) * ls]; tw: N-entry table
- * exp(-2 pi i k/N).
+ * buffer, element idx at lds[padi<PAD>(idx) * ls]; twf: twiddle source.
  */
-template <class T, int N, int E, int R, int Ns, int DIR, bool LAST, int PAD = 0>
-MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+template <class T, int N, int E, int R, int Ns, int DIR, bool LAST, int PAD, class TwF>
+MPC_HD void fft_stage_t(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const TwF &twf) {
     constexpr int TPL = N / E;
     constexpr int B = E / R;
     static_assert(E % R == 0, "radix must divide elements per thread");
@@ -210,7 +247,7 @@ MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *t
         for (int r = 0; r < R; ++r) u[r] = v[b + r * B];
         if (Ns > 1) {
             cx<T> w[R];
-            stage_twiddles<R, DIR>(w, tw, k * (N / (Ns * R)));
+            twf(b, k, w);
 #pragma unroll
             for (int r = 1; r < R; ++r) u[r] = cmul(u[r], w[r]);
         }
@@ -225,6 +262,32 @@ MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *t
         }
     }
 }
+/* a stage with twiddles from the N-entry table exp(-2 pi i k/N) */
+template <class T, int N, int E, int R, int Ns, int DIR, bool LAST, int PAD = 0>
+MPC_HD void fft_stage(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
+    fft_stage_t<T, N, E, R, Ns, DIR, LAST, PAD>(v, tid, lds, ls, TwFromTable<T, N, R, Ns, DIR>{tw});
+}
+
+/*
+ * Per-thread twiddle powers of every stage of a plan (radices Rs, first
+ * stage at group size Ns), loaded once with load(tw, tid) and reused by
+ * every transform of that length the thread runs, in either direction.
+ */
+template <class T, int N, int E, int Ns, int... Rs> struct TwChain {
+    MPC_HD void load(const cx<T> *, int) {}
+};
+template <class T, int N, int E, int Ns, int R, int... Rest> struct TwChain<T, N, E, Ns, R, Rest...> {
+    static constexpr int B = E / R, NP = tw_npow<R>(), TPL = N / E;
+    cx<T> p[B][NP];
+    TwChain<T, N, E, Ns * R, Rest...> next;
+    MPC_HD void load(const cx<T> *tw, int tid) {
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) load_powers<R>(p[b], tw, ((tid + TPL * b) % Ns) * (N / (Ns * R)));
+        }
+        next.load(tw, tid);
+    }
+};
 
 template <class T, int N, int E, int PAD = 0>
 MPC_HD void fft_gather(cx<T> (&v)[E], int tid, const cx<T> *lds, int ls) {

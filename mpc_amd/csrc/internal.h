@@ -47,7 +47,7 @@ struct PlasmaK {
 template <class T> struct TArgs {
     cx<T> *field;
     const cx<T> *tw;     /* Nt twiddles exp(-2 pi i k / Nt) */
-    const cx<T> *P;      /* pre table, FFT order, includes 1/Nt */
+    const cx<T> *P;      /* pre table, FFT order, includes 1/Nt; nullptr: already applied */
     const cx<T> *Q;      /* post table (only with out_time) or nullptr */
     const T *wfft;       /* FFT-order Omega grid */
     double *rho_block;   /* per-block max(ions1 + 2 ions2), may be nullptr */
@@ -65,11 +65,16 @@ template <class T> struct AArgs {
     long s_line;         /* element stride along the line */
     long s_outer;        /* stride between tiles' outer index */
     long m_line, m_outer;/* mirror index = i*m_line + outer*m_outer */
+    const cx<T> *Pt;     /* per-t factor applied before the store (t-spectral field), or nullptr */
     int nt;              /* contiguous t extent */
+    int n_outer;         /* number of lines per t value */
+    int tg;              /* tile order: runs of tg t-chunks, then the outer index */
 };
 
 /* launchers (kernels.hip) */
-template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s);
+/* nblocks (may be nullptr) receives the grid size, i.e. how many rho_block
+ * entries the launch writes */
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks);
 template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s);
 int tline_variants(void);
 int axis_variants(void);
@@ -113,6 +118,7 @@ struct mpc_ctx_s {
     void *wfft = nullptr;            /* FFT-order Omega, precision T */
     void *mirror = nullptr;          /* mirror table [y][x] */
     double *rho_block = nullptr;     /* per t-line block */
+    int rho_nvalid = 0;              /* entries written by the last plasma t-line launch */
     long rho_nblocks = 0;
     double *diag_part = nullptr;     /* diagnostics partials */
     double *diag_line = nullptr;     /* per-line power */
@@ -122,7 +128,7 @@ struct mpc_ctx_s {
     mpc::PlasmaK pk{};
     std::map<std::string, void *> tables;   /* table cache keyed by kind+coef */
     int domain_freq = 0;             /* resident field is F_t(A) (unnormalised) */
-    int tvar = 0, avar = 0;          /* kernel variants (autotuner / MPC_*_VARIANT) */
+    int tvar = 0, avar = 0, avar_y = 0; /* kernel variants: t-line, x-line, y-line (autotuner / MPC_*_VARIANT) */
     int tuned = 0;
     /* the resident field holds A / ascale; ascale is a power of two chosen
      * at upload so |A / ascale| <= 1 (photon-flux amplitudes reach 1e18 and

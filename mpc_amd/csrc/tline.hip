@@ -1,0 +1,278 @@
+/*
+ * tline.hip -- the fused t-line sweep of the split-step propagator.
+ *
+ * k_tline: a workgroup owns whole t-lines (layout A[y][x][t], t fastest).
+ * Per line: [t-FFT] x P(omega) -> t-IFFT of A and of -i omega A (spectral
+ * d/dt) -> Kerr + self-steepening + plasma (ADK + avalanche, reverse-t affine
+ * scan in FP64) -> A exp(dz A_nl) -> t-FFT [x Q -> t-IFFT].  Replaces
+ * propagation.c:164-327 and plasma.c:118-246 (which the reference runs on
+ * the host) and the t part of linearstep (propagation.c:11-30, 67-76).
+ *
+ * Built with machine-level loop-invariant code motion off: in the persistent
+ * form the line loop would otherwise hoist every FP64 polynomial constant of
+ * exp/log into a register pair for the whole kernel, which costs a wave per
+ * SIMD of occupancy.
+ */
+#include "device.h"
+
+namespace mpc {
+
+/* line FFT with twiddle powers cached per thread (TwChain of the plan) */
+template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
+struct DevFftC {
+    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
+                                               const TwChain<T, N, E, Ns, R, Rest...> &ch) {
+        constexpr bool last = sizeof...(Rest) == 0;
+        fft_stage_t<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds, ls, TwFromPowers<T, R, E / R, DIR>{ch.p});
+        if constexpr (!last) {
+            __syncthreads();
+            fft_gather<T, N, E, PAD>(v, tid, lds, ls);
+            __syncthreads();
+            DevFftC<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, ch.next);
+        }
+    }
+};
+template <class T, int N, int E, class Rad> struct ChainOf;
+template <class T, int N, int E, int... Rs> struct ChainOf<T, N, E, Radices<Rs...>> {
+    using type = TwChain<T, N, E, 1, Rs...>;
+    template <int DIR, int PAD>
+    static __device__ __forceinline__ void fft(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const type &ch) {
+        DevFftC<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, ch);
+    }
+};
+struct NoChain {};
+/* mark every cached twiddle power as redefined here, so products of them are
+ * formed where a stage uses them instead of being hoisted out of the line
+ * loop and held in registers */
+template <class T, int N, int E, int Ns> __device__ __forceinline__ void opaque_chain(TwChain<T, N, E, Ns> &) {}
+template <class T, int N, int E, int Ns, int R, int... Rest>
+__device__ __forceinline__ void opaque_chain(TwChain<T, N, E, Ns, R, Rest...> &ch) {
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int b = 0; b < E / R; ++b)
+#pragma unroll
+            for (int i = 0; i < tw_npow<R>(); ++i) asm volatile("" : "+v"(ch.p[b][i].re), "+v"(ch.p[b][i].im));
+    }
+    opaque_chain(ch.next);
+}
+
+/*
+ * k_tline<T, G, PLASMA, PERSIST>: a workgroup owns G::LPB t-lines at a time.
+ * With PERSIST the grid is what fits on the device at once and each
+ * workgroup walks line groups blockIdx.x, + gridDim.x, ...; it loads the
+ * twiddle powers and the Omega grid once, and loads the next group's field
+ * while it works on the current one, so the HBM latency of a line is hidden
+ * behind the previous line's FFTs.  Without PERSIST the grid has one
+ * workgroup per line group and every table is read per line.
+ */
+template <class T, class G, bool PLASMA, bool PERSIST>
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS)) k_tline(TArgs<T> a) {
+    using C = TCfg<T, G, PLASMA>;
+    using PL = typename G::PL;
+    using CO = ChainOf<T, G::NT, C::E, typename PL::R>;
+    constexpr int NT = G::NT;
+    constexpr int E = C::E, TPL = C::TPL, LPB = C::LPB;
+    __shared__ __align__(16) unsigned char smem[LPB * C::LINE_BYTES];
+    const int lslot0 = threadIdx.x / TPL, tid0 = threadIdx.x % TPL;
+    const long ngroups = a.nlines / LPB;
+    long grp = blockIdx.x;
+    if (grp >= ngroups) return;                       /* whole workgroup */
+    int tid = tid0, lslot = lslot0;
+
+    std::conditional_t<PERSIST, typename CO::type, NoChain> twc;
+    T wk[E];
+    if constexpr (PERSIST) {
+        twc.load(a.tw, tid);
+        if (a.do_nl && a.sst_on) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) wk[m] = a.wfft[tid + TPL * m];
+        }
+    }
+    unsigned char *lraw;
+    cx<T> *L;
+    auto fft = [&](cx<T>(&x)[E], auto dir) {
+        constexpr int DIR = decltype(dir)::value;
+        if constexpr (PERSIST) CO::template fft<DIR, G::PAD>(x, tid, L, 1, twc);
+        else line_fft<T, NT, PL, DIR, G::PAD>(x, tid, L, 1, a.tw);
+    };
+    using Fwd = std::integral_constant<int, -1>;
+    using Inv = std::integral_constant<int, +1>;
+
+    cx<T> nxt[E];
+    {
+        const cx<T> *g0 = a.field + (grp * LPB + lslot) * NT;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = g0[tid + TPL * m];
+    }
+    double rmax = 0.0;
+    for (;;) {
+        /* tid and lslot re-enter as opaque values each iteration, so the
+         * per-thread index arithmetic is recomputed where it is used rather
+         * than hoisted out of the loop and held in registers throughout */
+        tid = tid0;
+        lslot = lslot0;
+        asm volatile("" : "+v"(tid), "+v"(lslot));
+        if constexpr (PERSIST) opaque_chain(twc);
+        lraw = smem + lslot * C::LINE_BYTES;
+        L = reinterpret_cast<cx<T> *>(lraw);
+        cx<T> *g = a.field + (grp * LPB + lslot) * NT;
+        cx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        if (!a.in_freq) fft(v, Fwd{});
+        if (a.P) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
+        }
+        const long gnext = grp + gridDim.x;
+        if constexpr (PERSIST) {
+            if (gnext < ngroups) {
+                const cx<T> *gn = a.field + (gnext * LPB + lslot) * NT;
+#pragma unroll
+                for (int m = 0; m < E; ++m) nxt[m] = gn[tid + TPL * m];
+            }
+        }
+
+        if (a.do_nl) {
+            cx<T> d[E];
+            if (a.sst_on) {
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const T w = PERSIST ? wk[m] : a.wfft[tid + TPL * m];
+                    d[m] = mk(w * v[m].im, -w * v[m].re);   /* -i omega A(omega) */
+                }
+                fft(d, Inv{});                             /* dA */
+            }
+            fft(v, Inv{});                                 /* A  */
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
+                if (a.sst_on) {
+                    /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232) */
+                    const cx<T> t1 = cmulc(d[m], v[m]);
+                    const cx<T> t2 = cmulc(v[m], d[m]);
+                    const T sre = a.sst_coef * ((T)2 * t1.re + t2.re);
+                    const T sim = a.sst_coef * ((T)2 * t1.im + t2.im);
+                    d[m] = mk(-sre, a.gamma * a2 - sim);
+                } else {
+                    d[m] = mk((T)0, a.gamma * a2);
+                }
+            }
+            if constexpr (PLASMA) {
+                const double half_rho = 0.5 * a.pk.rho_nt;
+                plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
+                                   [&](int m, double F, double tau) {
+                                       /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
+                                       const double z = half_rho * F;
+                                       d[m].re -= (T)z;
+                                       d[m].im -= (T)(a.pk.w0 * tau * z);
+                                   });
+            }
+            /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const T er = exp_t<T>(a.dz * d[m].re);
+                T sn, cs;
+                sincos_t<T>(a.dz * d[m].im, &sn, &cs);
+                const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
+                v[m] = mk(er * z.re, er * z.im);
+            }
+            fft(v, Fwd{});
+        }
+        if (a.Q) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
+        }
+        if (a.out_time) fft(v, Inv{});
+#pragma unroll
+        for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
+        if (!PERSIST || gnext >= ngroups) break;
+        grp = gnext;
+    }
+    if constexpr (PLASMA) {
+        if (a.rho_block && a.do_nl) {
+            /* workgroup max of the ionisation fraction over its lines */
+            __shared__ double red[32];
+            double r = rmax;
+            for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_down(r, o));
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double mx = 0.0;
+                for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mx = fmax(mx, red[i]);
+                a.rho_block[blockIdx.x] = mx;
+            }
+        }
+    }
+}
+
+bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
+int tline_variants(void) { return 4; }
+
+/* workgroups resident on the device at once for kernel k (0: unknown) */
+static long resident_blocks(const void *k, int threads, int lds_bytes) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, lds_bytes) != hipSuccess) return 0;
+    return (long)ncu * per;
+}
+
+template <class T, class G, bool PL, bool PERSIST>
+static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
+    using C = TCfg<T, G, PL>;
+    if (a.nlines % C::LPB) return MPC_EINVAL;
+    const long ngroups = a.nlines / C::LPB;
+    long nb = ngroups;
+    if (PERSIST) {
+        static long resident = -1;              /* per instantiation: the same on every device of one model */
+        if (resident < 0) resident = resident_blocks((const void *)k_tline<T, G, PL, PERSIST>, C::THREADS, 0);
+        if (resident > 0 && resident < nb) nb = resident;
+    }
+    if (nb <= 0 || nb > 0x7fffffffL) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_tline<T, G, PL, PERSIST>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
+    if (nblocks) *nblocks = (int)nb;
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+/* variant = plan + 2 * mode.  plan 0: 8 elements per thread (TPlan), 1: 16
+ * per thread (Plan).  mode 0: persistent workgroups with cached twiddles
+ * and prefetch; 1: one workgroup per line group */
+template <class T, int NT, class GEO>
+static int tline_m(const TArgs<T> &a, bool plasma, bool persist, hipStream_t s, int *nb) {
+    if (persist)
+        return plasma ? tline_go<T, GEO, true, true>(a, s, nb) : tline_go<T, GEO, false, true>(a, s, nb);
+    return plasma ? tline_go<T, GEO, true, false>(a, s, nb) : tline_go<T, GEO, false, false>(a, s, nb);
+}
+template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, int variant, hipStream_t s, int *nb) {
+    const bool persist = (variant / 2) == 0;
+    if (variant % 2 == 1) return tline_m<T, NT, TGeo<NT, Plan<NT>>>(a, plasma, persist, s, nb);
+    return tline_m<T, NT, TGeo<NT>>(a, plasma, persist, s, nb);
+}
+template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks) {
+    if (variant < 0 || variant >= tline_variants()) return MPC_EINVAL;
+    switch (nt) {
+    case 512:  return tline_v<T, 512>(a, plasma, variant, s, nblocks);
+    case 1024: return tline_v<T, 1024>(a, plasma, variant, s, nblocks);
+    case 2048: return tline_v<T, 2048>(a, plasma, variant, s, nblocks);
+    case 4096: return tline_v<T, 4096>(a, plasma, variant, s, nblocks);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_tline<double>(const TArgs<double> &, int, bool, int, hipStream_t, int *);
+template int launch_tline<float>(const TArgs<float> &, int, bool, int, hipStream_t, int *);
+
+/* t-line kernel block geometry: the smallest lines-per-block over the
+ * variants, so per-block buffers sized by it fit every variant's grid */
+int tline_lines_per_block(int nt, bool plasma, int prec) {
+    (void)plasma; (void)prec;
+    auto mn = [](int a, int b) { return a < b ? a : b; };
+    switch (nt) {
+    case 512: return mn(TGeo<512>::LPB, TGeo<512, Plan<512>>::LPB);
+    case 1024: return mn(TGeo<1024>::LPB, TGeo<1024, Plan<1024>>::LPB);
+    case 2048: return mn(TGeo<2048>::LPB, TGeo<2048, Plan<2048>>::LPB);
+    case 4096: return mn(TGeo<4096>::LPB, TGeo<4096, Plan<4096>>::LPB);
+    default: return 1;
+    }
+}
+
+} // namespace mpc

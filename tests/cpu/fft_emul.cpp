@@ -35,7 +35,42 @@ template <class T, int N, int E, int DIR, int PAD, int... Rs> struct Unpack<T, N
     }
 };
 
-template <class T, int N, int DIR, class PL = Plan<N>, int PAD = 0> double check(unsigned seed) {
+/* the same transform with each thread's twiddle powers loaded once into a
+ * TwChain (the fused t-line kernel's form) */
+template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
+struct HostRunC {
+    using CH = TwChain<T, N, E, Ns, R, Rest...>;
+    static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const std::vector<const CH *> &ch) {
+        constexpr int TPL = N / E;
+        constexpr bool last = sizeof...(Rest) == 0;
+        for (int tid = 0; tid < TPL; ++tid) {
+            cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
+            fft_stage_t<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds.data(), 1,
+                                                         TwFromPowers<T, R, E / R, DIR>{ch[tid]->p});
+        }
+        if constexpr (!last) {
+            for (int tid = 0; tid < TPL; ++tid) {
+                cx<T>(&v)[E] = *reinterpret_cast<cx<T>(*)[E]>(&V[(size_t)tid * E]);
+                fft_gather<T, N, E, PAD>(v, tid, lds.data(), 1);
+            }
+            std::vector<const TwChain<T, N, E, Ns * R, Rest...> *> nx;
+            for (int tid = 0; tid < TPL; ++tid) nx.push_back(&ch[tid]->next);
+            HostRunC<T, N, E, DIR, PAD, Ns * R, Rest...>::run(V, lds, nx);
+        }
+    }
+};
+template <class T, int N, int E, int DIR, int PAD, class Rad> struct UnpackC;
+template <class T, int N, int E, int DIR, int PAD, int... Rs> struct UnpackC<T, N, E, DIR, PAD, Radices<Rs...>> {
+    static void run(std::vector<cx<T>> &V, std::vector<cx<T>> &lds, const cx<T> *tw) {
+        constexpr int TPL = N / E;
+        std::vector<TwChain<T, N, E, 1, Rs...>> chains(TPL);
+        std::vector<const TwChain<T, N, E, 1, Rs...> *> ptr;
+        for (int tid = 0; tid < TPL; ++tid) { chains[tid].load(tw, tid); ptr.push_back(&chains[tid]); }
+        HostRunC<T, N, E, DIR, PAD, 1, Rs...>::run(V, lds, ptr);
+    }
+};
+
+template <class T, int N, int DIR, class PL = Plan<N>, int PAD = 0, bool CACHED = false> double check(unsigned seed) {
     constexpr int E = PL::E, TPL = N / E;
     std::vector<cx<T>> tw(N), lds(N + (PAD ? N / PAD : 0)), V((size_t)TPL * E);
     for (int k = 0; k < N; ++k) {
@@ -47,7 +82,8 @@ template <class T, int N, int DIR, class PL = Plan<N>, int PAD = 0> double check
     for (int i = 0; i < N; ++i) { xr[i] = (T)(rand() / (double)RAND_MAX - 0.5); xi[i] = (T)(rand() / (double)RAND_MAX - 0.5); }
     for (int tid = 0; tid < TPL; ++tid)
         for (int m = 0; m < E; ++m) V[(size_t)tid * E + m] = mk((T)xr[tid + TPL * m], (T)xi[tid + TPL * m]);
-    Unpack<T, N, E, DIR, PAD, typename PL::R>::run(V, lds, tw.data());
+    if (CACHED) UnpackC<T, N, E, DIR, PAD, typename PL::R>::run(V, lds, tw.data());
+    else Unpack<T, N, E, DIR, PAD, typename PL::R>::run(V, lds, tw.data());
     double num = 0, den = 0;
     for (int k = 0; k < N; ++k) {
         long double sr = 0, si = 0;
@@ -78,7 +114,15 @@ template <int N> int run_t() {
     double e1 = check<double, N, -1, TPlan<N>, 8>(N), e2 = check<double, N, +1, TPlan<N>, 8>(N + 1);
     double e3 = check<float, N, -1, TPlan<N>, 8>(N + 2), e4 = check<float, N, +1, TPlan<N>, 8>(N + 3);
     printf("t-plan N=%5d f64 fwd %.3e inv %.3e  f32 fwd %.3e inv %.3e\n", N, e1, e2, e3, e4);
-    return (e1 > 1e-14 || e2 > 1e-14 || e3 > 2e-6 || e4 > 2e-6) ? 1 : 0;
+    int bad = (e1 > 1e-14 || e2 > 1e-14 || e3 > 2e-6 || e4 > 2e-6) ? 1 : 0;
+    /* cached twiddle powers, both t plans */
+    double c1 = check<double, N, -1, TPlan<N>, 8, true>(N), c2 = check<double, N, +1, TPlan<N>, 8, true>(N + 1);
+    double c3 = check<float, N, -1, Plan<N>, 16, true>(N + 2), c4 = check<float, N, +1, Plan<N>, 16, true>(N + 3);
+    printf("cached N=%5d f64 fwd %.3e inv %.3e  f32 (E=16) fwd %.3e inv %.3e\n", N, c1, c2, c3, c4);
+    if (c1 > 1e-14 || c2 > 1e-14 || c3 > 2e-6 || c4 > 2e-6) bad = 1;
+    /* cached and table twiddles give the same bits */
+    if (c1 != e1 || c2 != e2) { printf("cached twiddles differ from table twiddles at N=%d\n", N); bad = 1; }
+    return bad;
 }
 
 int main() {

@@ -178,21 +178,25 @@ def test_cfg3_fp32_precision_form(gpu_lib):
 
 @pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
 def test_every_kernel_variant_matches_oracle(gpu_lib, prec, tol):
-    """the autotuner's choices (FFT plan, tile, elements per thread) change speed only"""
-    from mpc_amd import lib
+    """the autotuner's choices (FFT plan, tile, elements per thread, tile
+    order; x and y chosen separately) change speed only"""
+    from mpc_amd import variant_counts
+    nt_var, na_var = variant_counts()
+    assert nt_var == 4 and na_var == 16
     inp, A0 = _case(64, 1024, plasm=1, sst_on=1, steps_per_pass=2, passes=2)
     ref, _ = O.run_fixed(inp, A0, 0, 3)
-    outs = {}
-    for tv in range(2):
-        for av in range(4):
-            print(f"case variant tline={tv} axis={av} prec={prec}", flush=True)
-            with Propagator(inp, prec) as p:
-                p.upload(A0)
-                p.set_variants(tv, av)
-                p.run(0, 3)
-                outs[(tv, av)] = p.download()
-            assert O.rel_l2(outs[(tv, av)], ref) <= tol
+    combos = [(tv, av, av) for tv in range(nt_var) for av in range(na_var)]
+    combos += [(0, 2, 7), (1, 13, 4)]          # x and y on different variants
+    for tv, xv, yv in combos:
+        print(f"case variant tline={tv} x={xv} y={yv} prec={prec}", flush=True)
+        with Propagator(inp, prec) as p:
+            p.upload(A0)
+            p.set_variants(tv, xv, yv)
+            assert p.variants() == (tv, xv, yv)
+            p.run(0, 3)
+            out = p.download()
+        assert O.rel_l2(out, ref) <= tol, (tv, xv, yv)
     with Propagator(inp, prec) as p:
         p.upload(A0)
-        tv, av = p.autotune()
-        assert 0 <= tv < 2 and 0 <= av < 4
+        tv, xv, yv = p.autotune()
+        assert 0 <= tv < nt_var and 0 <= xv < na_var and 0 <= yv < na_var
