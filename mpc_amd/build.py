@@ -31,10 +31,11 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wno-unused-result"]
 
-HIP_SOURCES = ["tline.hip", "kernels.hip", "api.cpp"]
-# per-file flags: the persistent t-line kernel keeps machine-level LICM from
-# hoisting exp/log polynomial constants out of its line loop into registers
-EXTRA_FLAGS = {"tline.hip": ["-mllvm", "-disable-machine-licm"]}
+HIP_SOURCES = ["tline.hip", "axis.hip", "kernels.hip", "api.cpp"]
+# per-file flags: the persistent sweep kernels keep machine-level LICM from
+# hoisting constants and twiddle products out of their loops into registers
+EXTRA_FLAGS = {"tline.hip": ["-mllvm", "-disable-machine-licm"],
+               "axis.hip": ["-mllvm", "-disable-machine-licm"]}
 C_SOURCES = ["host.c"]
 
 

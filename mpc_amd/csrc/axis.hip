@@ -1,0 +1,191 @@
+/*
+ * axis.hip -- the x- and y-line sweeps of the split-step propagator.
+ *
+ * k_axis: FFT along x or y of a tile of G consecutive t values (layout
+ * A[y][x][t], t fastest, so a tile row is G*sizeof(complex) contiguous
+ * bytes), x D(k), inverse FFT [x mirror] [x per-t factor].  Replaces the x/y
+ * parts of linearstep (propagation.c:31-57, 77-100); the diffraction operator
+ * is separable, so x and y each take one sweep.
+ *
+ * Built with machine-level loop-invariant code motion off, like tline.hip:
+ * the persistent form's tile loop would otherwise keep every product of the
+ * cached twiddle powers in registers.
+ */
+#include "device.h"
+
+namespace mpc {
+
+template <class T> struct ATile { static constexpr int G = 256 / (int)sizeof(cx<T>); };  /* 256 B rows */
+int axis_tile(int precision) { return precision == MPC_F32 ? ATile<float>::G : ATile<double>::G; }
+
+/* 8-elements-per-thread plans for the transverse lines */
+template <int N> struct APlan8 { using P = Plan<N>; static constexpr int E = P::E; using R = typename P::R; };
+template <> struct APlan8<512> { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
+template <> struct APlan8<256> { static constexpr int E = 8; using R = Radices<8, 8, 4>; };
+template <> struct APlan8<128> { static constexpr int E = 8; using R = Radices<8, 8, 2>; };
+template <> struct APlan8<64>  { static constexpr int E = 8; using R = Radices<8, 8>; };
+template <> struct APlan8<16>  { static constexpr int E = 8; using R = Radices<8, 2>; };
+
+template <class T, int N, class PL, int G_> struct ACfg {
+    static constexpr int E = PL::E, TPL = N / E, G = G_;
+    static constexpr int THREADS = G * TPL;
+};
+
+/*
+ * Tile lin -> (t-chunk, outer): runs of tg consecutive t-chunks, then the
+ * outer index, then the next run.  tg = all chunks walks t fastest; a small
+ * tg spreads the concurrently running tiles over the outer index.
+ */
+template <class T> __device__ __forceinline__ void axis_tile_of(const AArgs<T> &a, long lin, long &outer, long &t0, int G) {
+    const long rest = lin / a.tg;
+    outer = rest % a.n_outer;
+    t0 = ((rest / a.n_outer) * a.tg + lin % a.tg) * G;
+}
+
+/*
+ * With PERSIST the grid is what fits on the device at once; each workgroup
+ * walks tiles blockIdx.x, + gridDim.x, ..., keeps its twiddle powers and its
+ * D(k) entries in registers, and loads the next tile (and its per-t factor)
+ * while it transforms the current one.  Without PERSIST there is one
+ * workgroup per tile and the tables are read per tile.
+ */
+template <class T, int N, class PL, int G_, bool PERSIST>
+__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T> a) {
+    using C = ACfg<T, N, PL, G_>;
+    using CO = ChainOf<T, N, C::E, typename PL::R>;
+    constexpr int E = C::E, TPL = C::TPL, G = C::G;
+    __shared__ __align__(16) cx<T> lds[N * G];
+    const int gcol0 = threadIdx.x % G, tid0 = threadIdx.x / G;
+    const long ntiles = (long)(a.nt / G) * a.n_outer;
+    long lin = blockIdx.x;
+    if (lin >= ntiles) return;                        /* whole workgroup */
+
+    std::conditional_t<PERSIST, typename CO::type, NoChain> twc;
+    cx<T> Dc[E];
+    if constexpr (PERSIST) {
+        twc.load(a.tw, tid0);
+#pragma unroll
+        for (int m = 0; m < E; ++m) Dc[m] = a.D[tid0 + TPL * m];
+    }
+    int tid = tid0, gcol = gcol0;
+    auto fft = [&](cx<T>(&x)[E], auto dir) {
+        constexpr int DIR = decltype(dir)::value;
+        if constexpr (PERSIST) CO::template fft<DIR, 0>(x, tid, lds + gcol, G, twc);
+        else line_fft<T, N, PL, DIR>(x, tid, lds + gcol, G, a.tw);
+    };
+
+    cx<T> nxt[E], pn = mk((T)1, (T)0);
+    {
+        long outer, t0;
+        axis_tile_of(a, lin, outer, t0, G);
+        const cx<T> *b0 = a.field + outer * a.s_outer + t0 + gcol0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
+        if (a.Pt) pn = a.Pt[t0 + gcol0];
+    }
+    for (;;) {
+        /* tid and gcol re-enter as opaque values each iteration, so index
+         * arithmetic is recomputed where it is used instead of being held */
+        tid = tid0;
+        gcol = gcol0;
+        asm volatile("" : "+v"(tid), "+v"(gcol));
+        if constexpr (PERSIST) opaque_chain(twc);
+        long outer, t0;
+        axis_tile_of(a, lin, outer, t0, G);
+        cx<T> *base = a.field + outer * a.s_outer + t0 + gcol;
+        cx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        const cx<T> p = pn;
+        const long lnext = lin + gridDim.x;
+        if constexpr (PERSIST) {
+            if (lnext < ntiles) {
+                long on, tn;
+                axis_tile_of(a, lnext, on, tn, G);
+                const cx<T> *bn = a.field + on * a.s_outer + tn + gcol;
+#pragma unroll
+                for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * a.s_line];
+                if (a.Pt) pn = a.Pt[tn + gcol];
+            }
+        }
+        fft(v, std::integral_constant<int, -1>{});
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], PERSIST ? Dc[m] : a.D[tid + TPL * m]);
+        fft(v, std::integral_constant<int, +1>{});
+        if (a.M) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
+        }
+        if (a.Pt) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], p);
+        }
+#pragma unroll
+        for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
+        if (!PERSIST || lnext >= ntiles) break;
+        lin = lnext;
+    }
+}
+
+bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+int axis_variants(void) { return 32; }
+
+/* tile orders: runs of all / 1 / 4 / 16 t-chunks before the outer index moves */
+static int axis_tg(int order, int nchunks) {
+    static const int tg[4] = {0, 1, 4, 16};
+    const int g = tg[order & 3];
+    return (g == 0 || nchunks % g) ? nchunks : g;
+}
+template <class T, int N, class PL, int G, bool PERSIST>
+static int axis_go(AArgs<T> a, int outer, int order, hipStream_t s) {
+    using C = ACfg<T, N, PL, G>;
+    if (a.nt % G || outer <= 0) return MPC_EINVAL;
+    const int nch = a.nt / G;
+    a.n_outer = outer;
+    a.tg = axis_tg(order, nch);
+    long nb = (long)nch * outer;
+    if (PERSIST) {
+        static long resident = -1;              /* per instantiation: the same on every device of one model */
+        if (resident < 0) resident = resident_blocks((const void *)k_axis<T, N, PL, G, PERSIST>, C::THREADS, 0);
+        if (resident > 0 && resident < nb) nb = resident;
+    }
+    if (nb <= 0 || nb > 0x7fffffffL) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_axis<T, N, PL, G, PERSIST>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+/* variant = plan + 4 * order + 16 * mode.  plan 0: 16-or-more per thread
+ * (Plan), 256-B rows; 1: Plan, 128-B rows; 2: 8 per thread, 128-B rows; 3: 8
+ * per thread, 256-B rows.  order: see axis_tg.  mode 0: one workgroup per
+ * tile; 1: persistent workgroups with cached tables and prefetch. */
+template <class T, int N, int PERSIST>
+static int axis_p(const AArgs<T> &a, int outer, int plan, int order, hipStream_t s) {
+    constexpr int G = ATile<T>::G;
+    constexpr bool P = PERSIST != 0;
+    switch (plan) {
+    case 1: return axis_go<T, N, Plan<N>, G / 2, P>(a, outer, order, s);
+    case 2: return axis_go<T, N, APlan8<N>, G / 2, P>(a, outer, order, s);
+    case 3: return axis_go<T, N, APlan8<N>, G, P>(a, outer, order, s);
+    default: return axis_go<T, N, Plan<N>, G, P>(a, outer, order, s);
+    }
+}
+template <class T, int N> static int axis_v(const AArgs<T> &a, int outer, int variant, hipStream_t s) {
+    const int plan = variant % 4, order = (variant / 4) % 4;
+    return (variant / 16) ? axis_p<T, N, 1>(a, outer, plan, order, s) : axis_p<T, N, 0>(a, outer, plan, order, s);
+}
+template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s) {
+    if (variant < 0 || variant >= axis_variants()) return MPC_EINVAL;
+    switch (n) {
+    case 16:  return axis_v<T, 16>(a, outer, variant, s);
+    case 32:  return axis_v<T, 32>(a, outer, variant, s);
+    case 64:  return axis_v<T, 64>(a, outer, variant, s);
+    case 128: return axis_v<T, 128>(a, outer, variant, s);
+    case 192: return axis_v<T, 192>(a, outer, variant, s);
+    case 256: return axis_v<T, 256>(a, outer, variant, s);
+    case 512: return axis_v<T, 512>(a, outer, variant, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_axis<double>(const AArgs<double> &, int, int, int, hipStream_t);
+template int launch_axis<float>(const AArgs<float> &, int, int, int, hipStream_t);
+
+} // namespace mpc

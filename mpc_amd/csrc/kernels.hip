@@ -1,121 +1,13 @@
 /*
- * kernels.hip -- the transverse sweeps and auxiliary kernels of the
- * split-step propagator (the fused t-line sweep is in tline.hip).
- *
- *   k_axis   : FFT along x or y of a tile of G consecutive t values,
- *              x D(k), inverse FFT [x mirror] [x per-t factor].  Replaces
- *              the x/y parts of linearstep (propagation.c:31-57, 77-100);
- *              the diffraction operator is separable, so x and y each take
- *              one sweep.
- * plus layout transposes, diagnostics reductions, the adaptive-step error
- * and Richardson sweeps, and the plasma/ADK kernels behind the legacy
- * ioniz()/ADK() entry points.
+ * kernels.hip -- auxiliary kernels of the split-step propagator (the fused
+ * t-line sweep is in tline.hip, the x/y sweeps in axis.hip): layout
+ * transposes, diagnostics reductions, the adaptive-step error and Richardson
+ * sweeps, and the plasma/ADK kernels behind the legacy ioniz()/ADK() entry
+ * points.
  */
 #include "device.h"
 
 namespace mpc {
-
-/* ---------------------------------------------------------------------- */
-/* x / y line kernel                                                       */
-/* ---------------------------------------------------------------------- */
-template <class T> struct ATile { static constexpr int G = 256 / (int)sizeof(cx<T>); };  /* 256 B rows */
-int axis_tile(int precision) { return precision == MPC_F32 ? ATile<float>::G : ATile<double>::G; }
-
-/* 8-elements-per-thread plans for the transverse lines */
-template <int N> struct APlan8 { using P = Plan<N>; static constexpr int E = P::E; using R = typename P::R; };
-template <> struct APlan8<512> { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
-template <> struct APlan8<256> { static constexpr int E = 8; using R = Radices<8, 8, 4>; };
-template <> struct APlan8<128> { static constexpr int E = 8; using R = Radices<8, 8, 2>; };
-template <> struct APlan8<64>  { static constexpr int E = 8; using R = Radices<8, 8>; };
-template <> struct APlan8<16>  { static constexpr int E = 8; using R = Radices<8, 2>; };
-
-template <class T, int N, class PL, int G_> struct ACfg {
-    static constexpr int E = PL::E, TPL = N / E, G = G_;
-    static constexpr int THREADS = G * TPL;
-};
-
-template <class T, int N, class PL, int G_>
-__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T> a) {
-    using C = ACfg<T, N, PL, G_>;
-    constexpr int E = C::E, TPL = C::TPL, G = C::G;
-    __shared__ __align__(16) cx<T> lds[N * G];
-    const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
-    /* block -> (t-chunk, outer): runs of tg consecutive t-chunks, then the
-     * outer index, then the next run.  tg = all chunks walks t fastest; a
-     * small tg spreads the concurrently running tiles over the outer index. */
-    const int lin = blockIdx.x;
-    const int rest = lin / a.tg;
-    const long outer = rest % a.n_outer;
-    const long t0 = ((long)(rest / a.n_outer) * a.tg + lin % a.tg) * G;
-    cx<T> *base = a.field + outer * a.s_outer + t0 + gcol;
-    cx<T> v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = base[(long)(tid + TPL * m) * a.s_line];
-    line_fft<T, N, PL, -1>(v, tid, lds + gcol, G, a.tw);
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.D[tid + TPL * m]);
-    line_fft<T, N, PL, +1>(v, tid, lds + gcol, G, a.tw);
-    if (a.M) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
-    }
-    if (a.Pt) {
-        const cx<T> p = a.Pt[t0 + gcol];
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], p);
-    }
-#pragma unroll
-    for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
-}
-
-bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
-int axis_variants(void) { return 16; }
-
-/* tile orders: runs of all / 1 / 4 / 16 t-chunks before the outer index moves */
-static int axis_tg(int order, int nchunks) {
-    static const int tg[4] = {0, 1, 4, 16};
-    const int g = tg[order & 3];
-    return (g == 0 || nchunks % g) ? nchunks : g;
-}
-template <class T, int N, class PL, int G>
-static int axis_go(AArgs<T> a, int outer, int order, hipStream_t s) {
-    using C = ACfg<T, N, PL, G>;
-    if (a.nt % G || outer <= 0) return MPC_EINVAL;
-    const int nch = a.nt / G;
-    a.n_outer = outer;
-    a.tg = axis_tg(order, nch);
-    const long nb = (long)nch * outer;
-    if (nb > 0x7fffffffL) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_axis<T, N, PL, G>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
-}
-/* variant = plan + 4 * order.  plan 0: Plan, 256-B rows; 1: Plan, 128-B
- * rows; 2: 8 per thread, 128-B rows; 3: 8 per thread, 256-B rows.  order:
- * see axis_tg. */
-template <class T, int N> static int axis_v(const AArgs<T> &a, int outer, int variant, hipStream_t s) {
-    constexpr int G = ATile<T>::G;
-    const int order = variant / 4;
-    switch (variant % 4) {
-    case 1: return axis_go<T, N, Plan<N>, G / 2>(a, outer, order, s);
-    case 2: return axis_go<T, N, APlan8<N>, G / 2>(a, outer, order, s);
-    case 3: return axis_go<T, N, APlan8<N>, G>(a, outer, order, s);
-    default: return axis_go<T, N, Plan<N>, G>(a, outer, order, s);
-    }
-}
-template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s) {
-    switch (n) {
-    case 16:  return axis_v<T, 16>(a, outer, variant, s);
-    case 32:  return axis_v<T, 32>(a, outer, variant, s);
-    case 64:  return axis_v<T, 64>(a, outer, variant, s);
-    case 128: return axis_v<T, 128>(a, outer, variant, s);
-    case 192: return axis_v<T, 192>(a, outer, variant, s);
-    case 256: return axis_v<T, 256>(a, outer, variant, s);
-    case 512: return axis_v<T, 512>(a, outer, variant, s);
-    default: return MPC_EUNSUPP;
-    }
-}
-template int launch_axis<double>(const AArgs<double> &, int, int, int, hipStream_t);
-template int launch_axis<float>(const AArgs<float> &, int, int, int, hipStream_t);
 
 /* ---------------------------------------------------------------------- */
 /* layout conversion: reference A[t][y][x] (FP64) <-> internal A[y][x][t]  */

@@ -17,45 +17,6 @@
 
 namespace mpc {
 
-/* line FFT with twiddle powers cached per thread (TwChain of the plan) */
-template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
-struct DevFftC {
-    static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
-                                               const TwChain<T, N, E, Ns, R, Rest...> &ch) {
-        constexpr bool last = sizeof...(Rest) == 0;
-        fft_stage_t<T, N, E, R, Ns, DIR, last, PAD>(v, tid, lds, ls, TwFromPowers<T, R, E / R, DIR>{ch.p});
-        if constexpr (!last) {
-            __syncthreads();
-            fft_gather<T, N, E, PAD>(v, tid, lds, ls);
-            __syncthreads();
-            DevFftC<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, ch.next);
-        }
-    }
-};
-template <class T, int N, int E, class Rad> struct ChainOf;
-template <class T, int N, int E, int... Rs> struct ChainOf<T, N, E, Radices<Rs...>> {
-    using type = TwChain<T, N, E, 1, Rs...>;
-    template <int DIR, int PAD>
-    static __device__ __forceinline__ void fft(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const type &ch) {
-        DevFftC<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, ch);
-    }
-};
-struct NoChain {};
-/* mark every cached twiddle power as redefined here, so products of them are
- * formed where a stage uses them instead of being hoisted out of the line
- * loop and held in registers */
-template <class T, int N, int E, int Ns> __device__ __forceinline__ void opaque_chain(TwChain<T, N, E, Ns> &) {}
-template <class T, int N, int E, int Ns, int R, int... Rest>
-__device__ __forceinline__ void opaque_chain(TwChain<T, N, E, Ns, R, Rest...> &ch) {
-    if constexpr (Ns > 1) {
-#pragma unroll
-        for (int b = 0; b < E / R; ++b)
-#pragma unroll
-            for (int i = 0; i < tw_npow<R>(); ++i) asm volatile("" : "+v"(ch.p[b][i].re), "+v"(ch.p[b][i].im));
-    }
-    opaque_chain(ch.next);
-}
-
 /*
  * k_tline<T, G, PLASMA, PERSIST>: a workgroup owns G::LPB t-lines at a time.
  * With PERSIST the grid is what fits on the device at once and each
@@ -208,15 +169,6 @@ __global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS)) k_tline(TArgs<T
 
 bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
 int tline_variants(void) { return 4; }
-
-/* workgroups resident on the device at once for kernel k (0: unknown) */
-static long resident_blocks(const void *k, int threads, int lds_bytes) {
-    int dev = 0, ncu = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, lds_bytes) != hipSuccess) return 0;
-    return (long)ncu * per;
-}
 
 template <class T, class G, bool PL, bool PERSIST>
 static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
