@@ -17,7 +17,15 @@ namespace mpc {
 /* ---------------------------------------------------------------------- */
 /* FFT driver: unrolls Plan<N>::R into stages with LDS exchange between    */
 /* ---------------------------------------------------------------------- */
-template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
+/*
+ * BOFF = 0: one exchange buffer, two barriers per exchange.  BOFF != 0: the
+ * stages alternate between lds and lds + BOFF*ls, so a stage's writes can
+ * never meet a slower thread's reads of the previous exchange and one
+ * barrier per exchange suffices; the caller then puts a barrier before the
+ * transform (the buffer it writes first may still be read by the previous
+ * user of the LDS).  The sign of BOFF flips at every stage.
+ */
+template <class T, int N, int E, int DIR, int PAD, int BOFF, int Ns, int R, int... Rest>
 struct DevFft {
     static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
                                                const cx<T> *tw) {
@@ -26,21 +34,27 @@ struct DevFft {
         if constexpr (!last) {
             __syncthreads();
             fft_gather<T, N, E, PAD>(v, tid, lds, ls);
-            __syncthreads();
-            DevFft<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
+            if constexpr (BOFF == 0) {
+                __syncthreads();
+                DevFft<T, N, E, DIR, PAD, 0, Ns * R, Rest...>::run(v, tid, lds, ls, tw);
+            } else {
+                DevFft<T, N, E, DIR, PAD, -BOFF, Ns * R, Rest...>::run(v, tid, lds + BOFF * ls, ls, tw);
+            }
         }
     }
 };
-template <class T, int N, int E, int DIR, int PAD, class Rad> struct DevFftP;
-template <class T, int N, int E, int DIR, int PAD, int... Rs> struct DevFftP<T, N, E, DIR, PAD, Radices<Rs...>> {
+template <class T, int N, int E, int DIR, int PAD, int BOFF, class Rad> struct DevFftP;
+template <class T, int N, int E, int DIR, int PAD, int BOFF, int... Rs>
+struct DevFftP<T, N, E, DIR, PAD, BOFF, Radices<Rs...>> {
     static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
-        DevFft<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, tw);
+        if constexpr (BOFF != 0) __syncthreads();
+        DevFft<T, N, E, DIR, PAD, BOFF, 1, Rs...>::run(v, tid, lds, ls, tw);
     }
 };
 /* line FFT with plan PL (PL::E elements per thread, radix list PL::R) */
-template <class T, int N, class PL, int DIR, int PAD = 0>
+template <class T, int N, class PL, int DIR, int PAD = 0, int BOFF = 0>
 __device__ __forceinline__ void line_fft(cx<T> (&v)[PL::E], int tid, cx<T> *lds, int ls, const cx<T> *tw) {
-    DevFftP<T, N, PL::E, DIR, PAD, typename PL::R>::run(v, tid, lds, ls, tw);
+    DevFftP<T, N, PL::E, DIR, PAD, BOFF, typename PL::R>::run(v, tid, lds, ls, tw);
 }
 
 /* t-line geometry: plan, LDS padding and lines per 256-thread block.  The
@@ -59,14 +73,29 @@ template <int NT_, class PL_ = TPlan<NT_>> struct TGeo {
 /* ---------------------------------------------------------------------- */
 /* plasma model pieces (FP64 in both precisions)                           */
 /* ---------------------------------------------------------------------- */
-/* ADK rate Kon_a (Kon_b/Ea)^Kon_c exp(-Kon_b/(3 Ea)) (plasma.c:55-56) in log
- * form with e = Kon_b/(3 Ea) = ke * (1/s) and log(Kon_b/Ea) = L + log(1/s);
- * exactly 0 where the exponential underflows in double precision. */
+/*
+ * ADK rate Kon_a (Kon_b/Ea)^Kon_c exp(-Kon_b/(3 Ea)) (plasma.c:55-56) in log
+ * form with e = Kon_b/(3 Ea) = ke * (1/s) and log(Kon_b/Ea) = L + log(1/s).
+ * Rates with e > ADK_E_MAX are taken as 0: there Kon_b/Ea = 3e > 300, and
+ * for Kon_c < 4 the rate is below Kon_a * 300^4 * exp(-100) < 1e-33 Kon_a,
+ * so per step it adds less than 1e-30 to an ionised fraction.  That skips
+ * the exp/log in the field's wings, where it would only produce such values.
+ */
+constexpr double ADK_E_MAX = 100.0;
 __device__ __forceinline__ double adk_rate(double ka, double ke, double L, double kc, double invs) {
     const double e = ke * invs;
-    if (ka == 0.0 || e > 760.0) return 0.0;
+    if (ka == 0.0 || e > ADK_E_MAX) return 0.0;
     const double w = ka * exp(kc * (L + log(invs)) - e);
     return (w != w) ? 0.0 : w;
+}
+/* 1/d for normal d > 0: hardware estimate and two Newton steps (within an
+ * ulp or so of the correctly rounded quotient) */
+__device__ __forceinline__ double fast_rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
 }
 
 /* 1/s, s = sqrt(2 ZW0 I / NC) the field scale from |A|^2, with I floored at
@@ -82,11 +111,27 @@ __device__ __forceinline__ double plasma_invs(double a2, const PlasmaK &k, doubl
 __device__ __forceinline__ double plasma_sigma(double invs, const PlasmaK &k, double *tau) {
     const double tc = k.kt * invs;
     *tau = tc;
-    return k.K4 * tc / (1.0 + k.K5 * (tc * tc));
+    /* 1 + K5 tc^2 is in [1, 1e60] for any I >= the 1e-30 floor */
+    return k.K4 * tc * fast_rcp(1.0 + k.K5 * (tc * tc));
 }
-/* -expm1(-R): a 5-term series below 1e-4 (truncation < 1e-20 relative) */
+/* 1 - exp(-R) = sum_{k>=1} (-1)^(k+1) R^k / k!: twelve terms up to R = 0.1
+ * (truncation below 2e-22 relative), -expm1(-R) beyond */
 __device__ __forceinline__ double one_minus_exp_neg(double R) {
-    if (R < 1e-4) return R * (1.0 + R * (-0.5 + R * (1.0 / 6.0 + R * (-1.0 / 24.0 + R * (1.0 / 120.0)))));
+    if (R <= 0.1) {
+        double p = -1.0 / 479001600.0;                  /* -1/12! */
+        p = fma(p, R, 1.0 / 39916800.0);
+        p = fma(p, R, -1.0 / 3628800.0);
+        p = fma(p, R, 1.0 / 362880.0);
+        p = fma(p, R, -1.0 / 40320.0);
+        p = fma(p, R, 1.0 / 5040.0);
+        p = fma(p, R, -1.0 / 720.0);
+        p = fma(p, R, 1.0 / 120.0);
+        p = fma(p, R, -1.0 / 24.0);
+        p = fma(p, R, 1.0 / 6.0);
+        p = fma(p, R, -0.5);
+        p = fma(p, R, 1.0);
+        return p * R;
+    }
     return -expm1(-R);
 }
 
@@ -179,6 +224,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     double *U = lds, *X = lds + L::ARR, *scr = lds + 2 * L::ARR;
     int anyc = 0;
     double invs[E];
+    __syncthreads();              /* the LDS may still be read by the caller's last exchange */
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
@@ -191,7 +237,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
         U[padp<G>(k)] = 1.0 - a;
         X[padp<G>(k)] = b * (1.0 - a) + a;
-        anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= 760.0);
+        anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= ADK_E_MAX);
     }
     __syncthreads();
     /* U <- u = 1 - exp(-R) */
@@ -231,11 +277,18 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
 /* ---------------------------------------------------------------------- */
 /* t-line kernel                                                           */
 /* ---------------------------------------------------------------------- */
+/* per-line LDS: the FFT exchange buffer (doubled, BOFF != 0, while a
+ * workgroup's LDS stays within 80 KB, i.e. two workgroups per CU) unioned
+ * with the plasma arrays */
 template <class T, class G, bool PLASMA> struct TCfg {
     static constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB, THREADS = G::THREADS;
-    static constexpr int FFT_BYTES = G::FFT_ELEMS * (int)sizeof(cx<T>);
+    static constexpr int FFT1 = G::FFT_ELEMS * (int)sizeof(cx<T>);
     static constexpr int PL_BYTES = PLASMA ? PlasmaLds<G>::BYTES : 0;
+    static constexpr bool DB = LPB * 2 * FFT1 <= 80 * 1024;
+    static constexpr int BOFF = DB ? G::FFT_ELEMS : 0;
+    static constexpr int FFT_BYTES = (DB ? 2 : 1) * FFT1;
     static constexpr int LINE_BYTES = ((FFT_BYTES > PL_BYTES ? FFT_BYTES : PL_BYTES) + 15) / 16 * 16;
+    static_assert(LPB * LINE_BYTES <= 160 * 1024, "t-line workgroup exceeds the LDS");
 };
 
 /*
@@ -309,7 +362,7 @@ template <> __device__ __forceinline__ double exp_t<double>(double x) { return e
 template <> __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
 
 /* line FFT with twiddle powers cached per thread (TwChain of the plan) */
-template <class T, int N, int E, int DIR, int PAD, int Ns, int R, int... Rest>
+template <class T, int N, int E, int DIR, int PAD, int BOFF, int Ns, int R, int... Rest>
 struct DevFftC {
     static __device__ __forceinline__ void run(cx<T> (&v)[E], int tid, cx<T> *lds, int ls,
                                                const TwChain<T, N, E, Ns, R, Rest...> &ch) {
@@ -318,17 +371,23 @@ struct DevFftC {
         if constexpr (!last) {
             __syncthreads();
             fft_gather<T, N, E, PAD>(v, tid, lds, ls);
-            __syncthreads();
-            DevFftC<T, N, E, DIR, PAD, Ns * R, Rest...>::run(v, tid, lds, ls, ch.next);
+            if constexpr (BOFF == 0) {
+                __syncthreads();
+                DevFftC<T, N, E, DIR, PAD, 0, Ns * R, Rest...>::run(v, tid, lds, ls, ch.next);
+            } else {
+                DevFftC<T, N, E, DIR, PAD, -BOFF, Ns * R, Rest...>::run(v, tid, lds + BOFF * ls, ls, ch.next);
+            }
         }
     }
 };
 template <class T, int N, int E, class Rad> struct ChainOf;
 template <class T, int N, int E, int... Rs> struct ChainOf<T, N, E, Radices<Rs...>> {
     using type = TwChain<T, N, E, 1, Rs...>;
-    template <int DIR, int PAD>
+    /* line FFT with the thread's cached twiddle powers; BOFF as for DevFft */
+    template <int DIR, int PAD, int BOFF = 0>
     static __device__ __forceinline__ void fft(cx<T> (&v)[E], int tid, cx<T> *lds, int ls, const type &ch) {
-        DevFftC<T, N, E, DIR, PAD, 1, Rs...>::run(v, tid, lds, ls, ch);
+        if constexpr (BOFF != 0) __syncthreads();
+        DevFftC<T, N, E, DIR, PAD, BOFF, 1, Rs...>::run(v, tid, lds, ls, ch);
     }
 };
 struct NoChain {};

@@ -53,8 +53,8 @@ __global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS)) k_tline(TArgs<T
     cx<T> *L;
     auto fft = [&](cx<T>(&x)[E], auto dir) {
         constexpr int DIR = decltype(dir)::value;
-        if constexpr (PERSIST) CO::template fft<DIR, G::PAD>(x, tid, L, 1, twc);
-        else line_fft<T, NT, PL, DIR, G::PAD>(x, tid, L, 1, a.tw);
+        if constexpr (PERSIST) CO::template fft<DIR, G::PAD, C::BOFF>(x, tid, L, 1, twc);
+        else line_fft<T, NT, PL, DIR, G::PAD, C::BOFF>(x, tid, L, 1, a.tw);
     };
     using Fwd = std::integral_constant<int, -1>;
     using Inv = std::integral_constant<int, +1>;
