@@ -57,17 +57,16 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[PL::E], int tid, cx<T> *lds,
     DevFftP<T, N, PL::E, DIR, PAD, BOFF, typename PL::R>::run(v, tid, lds, ls, tw);
 }
 
-/* t-line geometry: plan, LDS padding and lines per 256-thread block.  The
- * LDS pad slot every E elements matches the stride of the first Stockham
- * stage's writes. */
+/* t-line geometry: plan, LDS layout (XOR swizzle keyed to E) and lines per
+ * 256-thread block */
 template <int NT_, class PL_ = TPlan<NT_>> struct TGeo {
     static constexpr int NT = NT_;
     using PL = PL_;
     static constexpr int E = PL::E, TPL = NT / E;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int THREADS = LPB * TPL;
-    static constexpr int PAD = E;
-    static constexpr int FFT_ELEMS = NT + NT / PAD;
+    static constexpr int PAD = -ilog2(E);              /* XOR swizzle, see padi */
+    static constexpr int FFT_ELEMS = padded_elems<PAD>(NT);
 };
 
 /* ---------------------------------------------------------------------- */
@@ -135,9 +134,10 @@ __device__ __forceinline__ double one_minus_exp_neg(double R) {
     return -expm1(-R);
 }
 
-/* plasma arrays: one pad slot per E-element segment, so the per-thread
- * consecutive segments of the scan start in different LDS banks */
-template <class G> __device__ __forceinline__ int padp(int i) { return i + i / G::E; }
+/* plasma arrays: the exchange buffers' swizzle, so both the per-element
+ * accesses (unit lane stride) and the scan's per-thread segments (lane
+ * stride E) are free of bank conflicts */
+template <class G> __device__ __forceinline__ int padp(int i) { return padi<G::PAD>(i); }
 
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
@@ -197,7 +197,7 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
 
 template <class G> struct PlasmaLds {
     static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
-    static constexpr int ARR = NT + NT / E;            /* padded doubles per array */
+    static constexpr int ARR = padded_elems<G::PAD>(NT);   /* doubles per array */
     static constexpr int SCR = 2 * NW + TPL;
     static constexpr int BYTES = (2 * ARR + SCR) * 8;
 };

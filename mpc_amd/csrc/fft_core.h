@@ -150,10 +150,24 @@ template <int DIR, class T> struct Dft<16, DIR, T> {
     static MPC_HD void run(cx<T> *x) { DftCT<4, 4, DIR, T>::run(x); }
 };
 
-/* LDS index with one padding slot every PAD elements (PAD = 0: none).  The
- * padding breaks the power-of-two strides of the early Stockham stages so
- * consecutive lanes land in different LDS banks. */
-template <int PAD> MPC_HD int padi(int i) { return PAD ? i + i / PAD : i; }
+/*
+ * LDS slot of element i of an exchange buffer.  PAD > 0: one padding slot
+ * every PAD elements.  PAD < 0: no padding; the low five bits of i are XORed
+ * with bits [s, s+5), s = -PAD.  With s = log2(elements per thread) the
+ * strided writes of the early Stockham stages (lane stride E) and the
+ * unit-stride gathers both hit distinct banks for every 16 or 32 lanes of
+ * 8-byte accesses, which padding cannot give for both at once.  The XOR form
+ * is a bijection on every multiple of 32 elements and stays inside it.
+ * PAD = 0: identity.
+ */
+template <int PAD> MPC_HD int padi(int i) {
+    if constexpr (PAD > 0) return i + i / PAD;
+    else if constexpr (PAD < 0) return i ^ ((i >> (-PAD)) & 31);
+    else return i;
+}
+/* elements an exchange buffer of n elements occupies under padi<PAD> */
+template <int PAD> constexpr int padded_elems(int n) { return PAD > 0 ? n + n / PAD : n; }
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
 /*
  * Stage twiddles w[r] = W^(r*idx) (conjugated for the inverse), r < R, from

@@ -26,8 +26,9 @@ namespace mpc {
  * behind the previous line's FFTs.  Without PERSIST the grid has one
  * workgroup per line group and every table is read per line.
  */
+/* at least two waves per SIMD: the register budget is 256 VGPRs */
 template <class T, class G, bool PLASMA, bool PERSIST>
-__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS)) k_tline(TArgs<T> a) {
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), 2) k_tline(TArgs<T> a) {
     using C = TCfg<T, G, PLASMA>;
     using PL = typename G::PL;
     using CO = ChainOf<T, G::NT, C::E, typename PL::R>;

@@ -72,7 +72,7 @@ template <class T, int N, int E, int DIR, int PAD, int... Rs> struct UnpackC<T, 
 
 template <class T, int N, int DIR, class PL = Plan<N>, int PAD = 0, bool CACHED = false> double check(unsigned seed) {
     constexpr int E = PL::E, TPL = N / E;
-    std::vector<cx<T>> tw(N), lds(N + (PAD ? N / PAD : 0)), V((size_t)TPL * E);
+    std::vector<cx<T>> tw(N), lds(padded_elems<PAD>(N)), V((size_t)TPL * E);
     for (int k = 0; k < N; ++k) {
         long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
         tw[k] = mk((T)cosl(a), (T)sinl(a));
@@ -115,9 +115,13 @@ template <int N> int run_t() {
     double e3 = check<float, N, -1, TPlan<N>, 8>(N + 2), e4 = check<float, N, +1, TPlan<N>, 8>(N + 3);
     printf("t-plan N=%5d f64 fwd %.3e inv %.3e  f32 fwd %.3e inv %.3e\n", N, e1, e2, e3, e4);
     int bad = (e1 > 1e-14 || e2 > 1e-14 || e3 > 2e-6 || e4 > 2e-6) ? 1 : 0;
+    /* the swizzled layout the t-line kernel uses */
+    double s1 = check<double, N, -1, TPlan<N>, -3>(N), s2 = check<float, N, +1, TPlan<N>, -3>(N + 3);
+    printf("swizzled N=%5d f64 fwd %.3e  f32 inv %.3e\n", N, s1, s2);
+    if (s1 != e1 || s2 != e4) { printf("swizzled layout changes the result at N=%d\n", N); bad = 1; }
     /* cached twiddle powers, both t plans */
     double c1 = check<double, N, -1, TPlan<N>, 8, true>(N), c2 = check<double, N, +1, TPlan<N>, 8, true>(N + 1);
-    double c3 = check<float, N, -1, Plan<N>, 16, true>(N + 2), c4 = check<float, N, +1, Plan<N>, 16, true>(N + 3);
+    double c3 = check<float, N, -1, Plan<N>, -4, true>(N + 2), c4 = check<float, N, +1, Plan<N>, -4, true>(N + 3);
     printf("cached N=%5d f64 fwd %.3e inv %.3e  f32 (E=16) fwd %.3e inv %.3e\n", N, c1, c2, c3, c4);
     if (c1 > 1e-14 || c2 > 1e-14 || c3 > 2e-6 || c4 > 2e-6) bad = 1;
     /* cached and table twiddles give the same bits */
