@@ -195,11 +195,16 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
     __syncthreads();
 }
 
+/* per-line plasma LDS: U and X, plus S (sigma) and V (1/s) while the
+ * workgroup's share stays within 80 KB; otherwise 1/s stays in registers and
+ * sigma is formed again for the result */
 template <class G> struct PlasmaLds {
     static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
     static constexpr int ARR = padded_elems<G::PAD>(NT);   /* doubles per array */
     static constexpr int SCR = 2 * NW + TPL;
-    static constexpr int BYTES = (2 * ARR + SCR) * 8;
+    static constexpr bool SV = G::LPB * (4 * ARR + SCR) * 8 <= 80 * 1024;
+    static constexpr int NARR = SV ? 4 : 2;
+    static constexpr int BYTES = (NARR * ARR + SCR) * 8;
 };
 
 /*
@@ -221,24 +226,35 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     using L = PlasmaLds<G>;
     constexpr int NT = G::NT;
     constexpr int E = L::E, TPL = L::TPL;
-    double *U = lds, *X = lds + L::ARR, *scr = lds + 2 * L::ARR;
+    double *U = lds, *X = lds + L::ARR, *scr = lds + L::NARR * L::ARR;
+    double *S = lds + 2 * L::ARR, *V = lds + 3 * L::ARR;   /* used when L::SV */
     int anyc = 0;
-    double invs[E];
+    double invs[L::SV ? 1 : E];
     __syncthreads();              /* the LDS may still be read by the caller's last exchange */
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = tid + TPL * m;
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
         double I;
-        invs[m] = plasma_invs(a2, pk, &I);
+        const double iv = plasma_invs(a2, pk, &I);
         double tc;
-        const double sig = plasma_sigma(invs[m], pk, &tc);
-        const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, invs[m]) * pk.Kon;
+        const double sig = plasma_sigma(iv, pk, &tc);
+        const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, iv) * pk.Kon;
         const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
         U[padp<G>(k)] = 1.0 - a;
         X[padp<G>(k)] = b * (1.0 - a) + a;
-        anyc |= (pk.kon2 != 0.0 && pk.ke2 * invs[m] <= ADK_E_MAX);
+        if constexpr (L::SV) {
+            S[padp<G>(k)] = sig;
+            V[padp<G>(k)] = iv;
+        } else {
+            invs[m] = iv;
+        }
+        anyc |= (pk.kon2 != 0.0 && pk.ke2 * iv <= ADK_E_MAX);
     }
+    auto inv_s = [&](int m) {
+        if constexpr (L::SV) return V[padp<G>(tid + TPL * m)];
+        else return invs[m];
+    };
     __syncthreads();
     /* U <- u = 1 - exp(-R) */
     affine_suffix_scan<G>(tid, scr,
@@ -248,7 +264,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     if (block_anyc) {
 #pragma unroll
         for (int m = 0; m < E; ++m)
-            X[padp<G>(tid + TPL * m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, invs[m]) * pk.Kon;
+            X[padp<G>(tid + TPL * m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, inv_s(m)) * pk.Kon;
         __syncthreads();
         /* X <- ions2: m_k = 1 - c_k, c'_k = c_k u_{k+1} */
         affine_suffix_scan<G>(tid, scr,
@@ -266,8 +282,13 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         const double i2 = block_anyc ? X[padp<G>(k)] : 0.0;
         const double i1 = U[padp<G>(k)] - i2;
         const double fr = i1 + 2.0 * i2;
-        double tc;
-        const double sig = plasma_sigma(invs[m], pk, &tc);
+        double tc, sig;
+        if constexpr (L::SV) {
+            sig = S[padp<G>(k)];
+            tc = pk.kt * V[padp<G>(k)];
+        } else {
+            sig = plasma_sigma(invs[m], pk, &tc);
+        }
         rho_max = fmax(rho_max, fr);
         sink(m, fr * sig, tc);
     }
