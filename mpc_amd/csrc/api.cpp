@@ -678,8 +678,9 @@ extern "C" int mpc_run_adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem
 /* ------------------------------------------------------------------------ */
 /* FFT / tile autotuner                                                      */
 /* ------------------------------------------------------------------------ */
-/* times every kernel variant (median of 3 launches) on a scratch copy of the
- * resident field and keeps the fastest; the choice only changes speed */
+/* times every kernel variant (median of 5 launches after a warm-up launch)
+ * on a scratch copy of the resident field and keeps the fastest; the choice
+ * only changes speed */
 template <class T> static int autotune(mpc_ctx *c) {
     const size_t bytes = c->esz * (size_t)c->N;
     if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
@@ -691,19 +692,21 @@ template <class T> static int autotune(mpc_ctx *c) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    auto time3 = [&](auto &&launch) -> float {
-        float best[3];
-        for (int r = 0; r < 3; r++) {
+    /* one untimed launch (first-use costs), then the median of five */
+    auto time_median = [&](auto &&launch) -> float {
+        if (launch() != 0) return 1e30f;
+        float t[5];
+        for (int r = 0; r < 5; r++) {
             (void)hipEventRecord(e0, c->stream);
             if (launch() != 0) return 1e30f;
             (void)hipEventRecord(e1, c->stream);
             (void)hipEventSynchronize(e1);
             float ms = 1e30f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            best[r] = ms;
+            t[r] = ms;
         }
-        std::sort(best, best + 3);
-        return best[1];
+        std::sort(t, t + 5);
+        return t[2];
     };
     const double dz = c->p.dzz;
     const cx<T> *D = get_table<T>(c, 'x', dz);
@@ -713,14 +716,14 @@ template <class T> static int autotune(mpc_ctx *c) {
     int vx = keep_a, vy = keep_y, vt = keep_t;
     for (int v = 0; !rc && v < axis_variants(); v++) {
         c->avar = c->avar_y = v;
-        const float mx = time3([&] { return sweep_axis<T>(c, 0, D, false); });
-        const float my = time3([&] { return sweep_axis<T>(c, 1, D, false); });
+        const float mx = time_median([&] { return sweep_axis<T>(c, 0, D, false); });
+        const float my = time_median([&] { return sweep_axis<T>(c, 1, D, false); });
         if (mx < bestX) { bestX = mx; vx = v; }
         if (my < bestY) { bestY = my; vy = v; }
     }
     for (int v = 0; !rc && v < tline_variants(); v++) {
         c->tvar = v;
-        const float ms = time3([&] { return sweep_t<T>(c, P, nullptr, true, true, false, dz); });
+        const float ms = time_median([&] { return sweep_t<T>(c, P, nullptr, true, true, false, dz); });
         if (ms < bestT) { bestT = ms; vt = v; }
     }
     (void)hipEventDestroy(e0);

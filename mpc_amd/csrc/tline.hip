@@ -26,9 +26,11 @@ namespace mpc {
  * behind the previous line's FFTs.  Without PERSIST the grid has one
  * workgroup per line group and every table is read per line.
  */
-/* at least two waves per SIMD: the register budget is 256 VGPRs */
+/* the FP32 persistent form is held to two waves per SIMD (256 VGPRs), which
+ * it meets with no more than a few spilled registers */
 template <class T, class G, bool PLASMA, bool PERSIST>
-__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), 2) k_tline(TArgs<T> a) {
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), (PERSIST && sizeof(T) == 4) ? 2 : 1)
+k_tline(TArgs<T> a) {
     using C = TCfg<T, G, PLASMA>;
     using PL = typename G::PL;
     using CO = ChainOf<T, G::NT, C::E, typename PL::R>;
@@ -169,7 +171,7 @@ __global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), 2) k_tline(TArg
 }
 
 bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
-int tline_variants(void) { return 4; }
+int tline_variants(void) { return 2; }
 
 template <class T, class G, bool PL, bool PERSIST>
 static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
@@ -187,19 +189,13 @@ static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
     if (nblocks) *nblocks = (int)nb;
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
-/* variant = plan + 2 * mode.  plan 0: 8 elements per thread (TPlan), 1: 16
- * per thread (Plan).  mode 0: persistent workgroups with cached twiddles
- * and prefetch; 1: one workgroup per line group */
-template <class T, int NT, class GEO>
-static int tline_m(const TArgs<T> &a, bool plasma, bool persist, hipStream_t s, int *nb) {
-    if (persist)
+/* variant 0: persistent workgroups with cached twiddles and prefetch;
+ * 1: one workgroup per line group.  Both use 8 elements per thread (TPlan). */
+template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, int variant, hipStream_t s, int *nb) {
+    using GEO = TGeo<NT>;
+    if (variant == 0)
         return plasma ? tline_go<T, GEO, true, true>(a, s, nb) : tline_go<T, GEO, false, true>(a, s, nb);
     return plasma ? tline_go<T, GEO, true, false>(a, s, nb) : tline_go<T, GEO, false, false>(a, s, nb);
-}
-template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, int variant, hipStream_t s, int *nb) {
-    const bool persist = (variant / 2) == 0;
-    if (variant % 2 == 1) return tline_m<T, NT, TGeo<NT, Plan<NT>>>(a, plasma, persist, s, nb);
-    return tline_m<T, NT, TGeo<NT>>(a, plasma, persist, s, nb);
 }
 template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks) {
     if (variant < 0 || variant >= tline_variants()) return MPC_EINVAL;
@@ -214,16 +210,14 @@ template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int 
 template int launch_tline<double>(const TArgs<double> &, int, bool, int, hipStream_t, int *);
 template int launch_tline<float>(const TArgs<float> &, int, bool, int, hipStream_t, int *);
 
-/* t-line kernel block geometry: the smallest lines-per-block over the
- * variants, so per-block buffers sized by it fit every variant's grid */
+/* t-line kernel block geometry (lines per workgroup; every variant's) */
 int tline_lines_per_block(int nt, bool plasma, int prec) {
     (void)plasma; (void)prec;
-    auto mn = [](int a, int b) { return a < b ? a : b; };
     switch (nt) {
-    case 512: return mn(TGeo<512>::LPB, TGeo<512, Plan<512>>::LPB);
-    case 1024: return mn(TGeo<1024>::LPB, TGeo<1024, Plan<1024>>::LPB);
-    case 2048: return mn(TGeo<2048>::LPB, TGeo<2048, Plan<2048>>::LPB);
-    case 4096: return mn(TGeo<4096>::LPB, TGeo<4096, Plan<4096>>::LPB);
+    case 512: return TGeo<512>::LPB;
+    case 1024: return TGeo<1024>::LPB;
+    case 2048: return TGeo<2048>::LPB;
+    case 4096: return TGeo<4096>::LPB;
     default: return 1;
     }
 }

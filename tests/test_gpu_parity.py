@@ -182,11 +182,11 @@ def test_every_kernel_variant_matches_oracle(gpu_lib, prec, tol):
     order; x and y chosen separately) change speed only"""
     from mpc_amd import variant_counts
     nt_var, na_var = variant_counts()
-    assert nt_var == 4 and na_var == 32
+    assert nt_var == 2 and na_var == 32
     inp, A0 = _case(64, 1024, plasm=1, sst_on=1, steps_per_pass=2, passes=2)
     ref, _ = O.run_fixed(inp, A0, 0, 3)
-    combos = [(tv, 0, 0) for tv in range(nt_var)] + [(2, av, av) for av in range(na_var)]
-    combos += [(0, 2, 7), (1, 13, 4), (3, 19, 30)]   # x and y on different variants
+    combos = [(tv, 0, 0) for tv in range(nt_var)] + [(1, av, av) for av in range(na_var)]
+    combos += [(0, 2, 7), (1, 13, 4), (0, 19, 30)]   # x and y on different variants
     for tv, xv, yv in combos:
         print(f"case variant tline={tv} x={xv} y={yv} prec={prec}", flush=True)
         with Propagator(inp, prec) as p:
