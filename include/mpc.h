@@ -115,6 +115,25 @@ int  mpc_set_variants_xy(mpc_ctx *ctx, int tline_variant, int x_variant, int y_v
 /* number of variants of each kind: t-line, and x/y-line (plan + 4 * tile order) */
 int  mpc_variant_counts(int *tline_variants, int *axis_variants);
 
+/*
+ * Batched runs (no counterpart in the reference): pulses of uneven grids,
+ * energies and media, one context each (same precision, steps_per_pass and
+ * npasses), advance together.  Every sweep of a step is one launch per size
+ * class covering all pulses of that class; with use_graph the launches of
+ * each pass are captured once into a HIP graph and replayed for every pass
+ * with the same launch list.  The x/y sweeps use each class's first
+ * pulse's plan and tile order (variant % 16) in the one-workgroup-per-tile
+ * form, the t sweep the one-workgroup-per-line-group form, so each pulse's
+ * result is bit-identical to mpc_run on its context with t-line variant 1
+ * and those x/y variants.  The batch does not own the contexts; destroy it
+ * before them.
+ */
+typedef struct mpc_batch_s mpc_batch;
+int  mpc_batch_create(mpc_ctx *const *ctxs, int n, mpc_batch **out);
+void mpc_batch_destroy(mpc_batch *b);
+int  mpc_batch_run(mpc_batch *b, int step0, int nsteps, int use_graph);
+int  mpc_batch_graphs(mpc_batch *b);   /* distinct pass graphs captured so far */
+
 /* Timing of the most recent mpc_run: average device time per launch of each
  * sweep kernel, measured with HIP events on the library's stream.
  * which: 0 = t-line, 1 = x-line, 2 = y-line.  Returns ms, or <0. */

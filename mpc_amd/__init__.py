@@ -186,3 +186,39 @@ class Propagator:
 
     def kernel_ms(self, which: int) -> float:
         return lib().mpc_kernel_ms(self.ctx, which)
+
+
+class Batch:
+    """Pulses advancing together (mpc_batch): every sweep of a step is one
+    launch per size class, and with graph=True each pass's launches are a
+    HIP graph captured once and replayed.  The propagators stay owned by the
+    caller; close the batch first."""
+
+    def __init__(self, props: list):
+        self.props = list(props)
+        arr = (C.c_void_p * len(self.props))(*[p.ctx.value for p in self.props])
+        self.b = C.c_void_p()
+        check(lib().mpc_batch_create(arr, len(self.props), C.byref(self.b)), "mpc_batch_create")
+
+    def run(self, step0: int, nsteps: int, graph: bool = True) -> None:
+        check(lib().mpc_batch_run(self.b, step0, nsteps, 1 if graph else 0), "mpc_batch_run")
+
+    def graphs(self) -> int:
+        return lib().mpc_batch_graphs(self.b)
+
+    def close(self):
+        if self.b:
+            lib().mpc_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -97,6 +97,10 @@ def lib() -> C.CDLL:
         "mpc_get_variants_xy": (C.c_int, [vp, iptr, iptr, iptr]),
         "mpc_set_variants_xy": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpc_variant_counts": (C.c_int, [iptr, iptr]),
+        "mpc_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+        "mpc_batch_destroy": (None, [vp]),
+        "mpc_batch_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mpc_batch_graphs": (C.c_int, [vp]),
         "mpc_mkinput": (C.c_int, [C.POINTER(MpcInputSpec), C.POINTER(MpcParams), dptr, dptr, dptr, dptr, vp]),
         "mpc_write_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),
         "mpc_read_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -32,7 +33,6 @@
 using namespace mpc;
 
 extern "C" int mpc_adk_constants(const char *gas, double *kon);
-namespace mpc { int tline_lines_per_block(int nt, bool plasma, int prec); }
 
 static int g_last_error = 0;
 
@@ -311,8 +311,9 @@ static void resolve_timing(mpc_ctx *c) {
     c->evn = 0;
 }
 
-template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror,
-                                         const cx<T> *Pt = nullptr) {
+/* kernel arguments of one x (axis 0) or y (axis 1) sweep over c's field */
+template <class T> static AArgs<T> axis_args(const mpc_ctx *c, int axis, const cx<T> *D, bool mirror,
+                                             const cx<T> *Pt) {
     AArgs<T> a{};
     a.field = (cx<T> *)c->field;
     a.tw = (const cx<T> *)c->tw_x;
@@ -329,6 +330,12 @@ template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, b
         a.s_outer = c->nt;
         if (mirror) { a.M = (const cx<T> *)c->mirror; a.m_line = c->nx; a.m_outer = 1; }
     }
+    return a;
+}
+
+template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror,
+                                         const cx<T> *Pt = nullptr) {
+    const AArgs<T> a = axis_args<T>(c, axis, D, mirror, Pt);
     tic(c, 1 + axis);
     const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx,
                                   axis == 0 ? c->avar : c->avar_y, c->stream);
@@ -346,8 +353,9 @@ template <class T> static int run_xy(mpc_ctx *c, double coef, bool mirror, const
     return sweep_axis<T>(c, 1, D, mirror, Pt);
 }
 
-template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q, bool in_freq, bool do_nl,
-                                      bool out_time, double dz) {
+/* kernel arguments of one t-line sweep over c's field */
+template <class T> static TArgs<T> tline_args(const mpc_ctx *c, const cx<T> *P, const cx<T> *Q, bool in_freq,
+                                              bool do_nl, bool out_time, double dz) {
     TArgs<T> a{};
     a.field = (cx<T> *)c->field;
     a.tw = (const cx<T> *)c->tw_t;
@@ -368,6 +376,13 @@ template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q
     a.nlines = c->nxy;
     const bool plasma = do_nl && c->p.plasm == 1;
     a.rho_block = plasma ? c->rho_block : nullptr;
+    return a;
+}
+
+template <class T> static int sweep_t(mpc_ctx *c, const cx<T> *P, const cx<T> *Q, bool in_freq, bool do_nl,
+                                      bool out_time, double dz) {
+    const TArgs<T> a = tline_args<T>(c, P, Q, in_freq, do_nl, out_time, dz);
+    const bool plasma = do_nl && c->p.plasm == 1;
     tic(c, 0);
     int nb = 0;
     const int rc = launch_tline<T>(a, c->nt, plasma, c->tvar, c->stream, &nb);
@@ -381,48 +396,320 @@ static int pass_sign(const mpc_ctx *c, double z) {
     return ((int)(std::floor(z / c->p.Cav) + 1) % 2) * 2 - 1;
 }
 
-template <class T> static int run_fixed(mpc_ctx *c, int step0, int nsteps) {
+/*
+ * One sweep of the fixed-step schedule.  kind 'x' / 'y': diffraction with the
+ * merged coefficient coef ('y' may carry the mirror and, on a t-spectral
+ * field, the t table of coefficient pcoef); 't': the fused t-line sweep with
+ * optional P (pcoef) and Q (dz/2) tables.
+ */
+struct SweepOp {
+    char kind = 'x';
+    double coef = 0.0;
+    bool mirror = false;
+    bool has_p = false, has_q = false;
+    double pcoef = 0.0;
+    bool in_freq = false, out_time = false;
+    double dz = 0.0;
+    int step = 0;
+};
+
+/*
+ * The schedule of mpc_run(step0, nsteps) on c as a list of sweeps: step n is
+ * L(dz/2) N(dz) L(dz/2) at z = (n + 1/2) dz; consecutive half steps merge,
+ * the field stays t-spectral between steps (the dispersion factor then rides
+ * on the y sweep), and the mirror is applied after the last step of each
+ * pass but the last.  *freq_end receives the domain the run leaves.
+ */
+static void plan_fixed(const mpc_ctx *c, int step0, int nsteps, std::vector<SweepOp> &ops, bool *freq_end) {
     const double dz = c->p.dzz;
     const int S = c->p.steps_per_pass > 0 ? c->p.steps_per_pass : 1;
     const int npass = c->p.npasses > 0 ? c->p.npasses : 1;
     double ct = 0.0, cxy = 0.0;
     bool freq = c->domain_freq != 0;
+    auto xy = [&](double coef, bool mirror, bool pfused, double pcoef, int n) {
+        if (coef == 0.0 && !mirror && !pfused) return;
+        SweepOp x;
+        x.kind = 'x'; x.coef = coef; x.step = n;
+        ops.push_back(x);
+        SweepOp y = x;
+        y.kind = 'y'; y.mirror = mirror; y.has_p = pfused; y.pcoef = pcoef;
+        ops.push_back(y);
+    };
     for (int n = step0; n < step0 + nsteps; n++) {
         const double z = ((double)n + 0.5) * dz;
         const int s = pass_sign(c, z);
         ct += dz / 2;
         cxy += s * dz / 2;
-        const cx<T> *P = get_table<T>(c, 't', ct);
-        if (!P) return MPC_ENOMEM;
         /* on a t-spectral field the dispersion factor rides on the y sweep's
          * store (the linear factors commute); the t-line kernel then skips it */
         const bool p_in_y = freq && cxy != 0.0;
-        RCHK(run_xy<T>(c, cxy, false, p_in_y ? P : nullptr));
-        if (p_in_y) P = nullptr;
+        xy(cxy, false, p_in_y, ct, n);
         cxy = 0.0;
         const bool last = (n == step0 + nsteps - 1);
         const bool mirror_after = ((n + 1) % S == 0) && ((n + 1) / S < npass);
+        SweepOp t;
+        t.kind = 't'; t.step = n; t.dz = dz; t.in_freq = freq;
+        t.has_p = !p_in_y; t.pcoef = ct;
         if (last) {
-            const cx<T> *Q = get_table<T>(c, 't', dz / 2);
-            if (!Q) return MPC_ENOMEM;
-            RCHK(sweep_t<T>(c, P, Q, freq, true, true, dz));
+            t.has_q = true; t.out_time = true;
             freq = false;
             ct = 0.0;
         } else {
-            RCHK(sweep_t<T>(c, P, nullptr, freq, true, false, dz));
             freq = true;
             ct = dz / 2;
         }
+        ops.push_back(t);
         cxy = s * dz / 2;
         if (mirror_after) {
-            RCHK(run_xy<T>(c, cxy, true));
+            xy(cxy, true, false, 0.0, n);
             cxy = 0.0;
         }
     }
-    if (cxy != 0.0) RCHK(run_xy<T>(c, cxy, false));
+    if (cxy != 0.0) xy(cxy, false, false, 0.0, step0 + nsteps - 1);
+    *freq_end = freq;
+}
+
+/* one planned sweep on c's own stream */
+template <class T> static int exec_op(mpc_ctx *c, const SweepOp &o) {
+    if (o.kind == 't') {
+        const cx<T> *P = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
+        const cx<T> *Q = o.has_q ? get_table<T>(c, 't', o.dz / 2) : nullptr;
+        if ((o.has_p && !P) || (o.has_q && !Q)) return MPC_ENOMEM;
+        return sweep_t<T>(c, P, Q, o.in_freq, true, o.out_time, o.dz);
+    }
+    const cx<T> *D = get_table<T>(c, 'x', o.coef);
+    const cx<T> *Pt = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
+    if (!D || (o.has_p && !Pt)) return MPC_ENOMEM;
+    return sweep_axis<T>(c, o.kind == 'x' ? 0 : 1, D, o.mirror, Pt);
+}
+
+template <class T> static int run_fixed(mpc_ctx *c, int step0, int nsteps) {
+    std::vector<SweepOp> ops;
+    bool freq = false;
+    plan_fixed(c, step0, nsteps, ops, &freq);
+    for (const SweepOp &o : ops) RCHK(exec_op<T>(c, o));
     c->domain_freq = freq ? 1 : 0;
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* batched runs: many pulses, one launch per sweep and size class, a HIP     */
+/* graph per pass                                                            */
+/* ------------------------------------------------------------------------ */
+struct mpc_batch_s {
+    std::vector<mpc_ctx *> ctx;
+    int prec = MPC_F64;
+    int S = 1;                                   /* steps per pass, common to all pulses */
+    hipStream_t stream = nullptr;
+    std::map<std::string, void *> pool;          /* device copies of descriptor arrays, by content */
+    struct Graph { hipGraph_t g = nullptr; hipGraphExec_t e = nullptr; };
+    std::map<std::string, Graph> graphs;         /* per-pass graphs, by launch list */
+};
+
+namespace {
+/* one batched launch: sweep kind, size class, kernel variant and the device
+ * descriptor / start arrays it reads */
+struct BLaunch {
+    char kind;
+    int len;            /* t length ('t') or line length ('x', 'y') */
+    int plasma;
+    int variant;
+    int n;              /* fields in the launch */
+    long nblocks;
+    const void *desc;
+    const long *starts;
+    int step;
+    std::vector<int> members;   /* batch indices, for the per-field bookkeeping */
+};
+}
+
+/* a device copy of bytes, shared by identical contents */
+static const void *pool_put(mpc_batch *b, const void *src, size_t bytes) {
+    std::string key((const char *)src, bytes);
+    auto it = b->pool.find(key);
+    if (it != b->pool.end()) return it->second;
+    void *d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(d); return nullptr; }
+    b->pool[key] = d;
+    return d;
+}
+
+/* the launches of op index i of every pulse's plan, grouped by size class */
+template <class T>
+static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &plans, size_t i,
+                       std::vector<BLaunch> &out) {
+    const char kind = plans[0][i].kind;
+    struct Cls { int len, plasma; std::vector<int> m; };
+    std::vector<Cls> cls;
+    for (int p = 0; p < (int)b->ctx.size(); p++) {
+        const mpc_ctx *c = b->ctx[p];
+        const int len = kind == 't' ? c->nt : (kind == 'x' ? c->nx : c->ny);
+        const int pl = kind == 't' ? (c->p.plasm == 1) : 0;
+        auto f = std::find_if(cls.begin(), cls.end(), [&](const Cls &k) { return k.len == len && k.plasma == pl; });
+        if (f == cls.end()) { cls.push_back({len, pl, {p}}); } else f->m.push_back(p);
+    }
+    for (const Cls &k : cls) {
+        BLaunch L{};
+        L.kind = kind; L.len = k.len; L.plasma = k.plasma; L.n = (int)k.m.size(); L.step = plans[0][i].step;
+        L.members = k.m;
+        std::vector<long> starts;
+        long tot = 0;
+        if (kind == 't') {
+            std::vector<TArgs<T>> d;
+            for (int p : k.m) {
+                mpc_ctx *c = b->ctx[p];
+                const SweepOp &o = plans[p][i];
+                const cx<T> *P = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
+                const cx<T> *Q = o.has_q ? get_table<T>(c, 't', o.dz / 2) : nullptr;
+                if ((o.has_p && !P) || (o.has_q && !Q)) return MPC_ENOMEM;
+                d.push_back(tline_args<T>(c, P, Q, o.in_freq, true, o.out_time, o.dz));
+                starts.push_back(tot);
+                tot += c->nxy / tline_lines_per_block(c->nt, true, b->prec);
+            }
+            L.desc = pool_put(b, d.data(), d.size() * sizeof(TArgs<T>));
+        } else {
+            const int axis = kind == 'x' ? 0 : 1;
+            const mpc_ctx *c0 = b->ctx[k.m[0]];
+            L.variant = (axis == 0 ? c0->avar : c0->avar_y) % 16;     /* one workgroup per tile */
+            const int G = axis_tile_g(b->prec, L.variant);
+            std::vector<AArgs<T>> d;
+            for (int p : k.m) {
+                mpc_ctx *c = b->ctx[p];
+                const SweepOp &o = plans[p][i];
+                const cx<T> *D = get_table<T>(c, 'x', o.coef);
+                const cx<T> *Pt = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
+                if (!D || (o.has_p && !Pt)) return MPC_ENOMEM;
+                if (c->nt % G) return MPC_EINVAL;
+                AArgs<T> a = axis_args<T>(c, axis, D, o.mirror, Pt);
+                const int nch = c->nt / G;
+                a.n_outer = axis == 0 ? c->ny : c->nx;
+                a.tg = axis_tile_tg(L.variant, nch);
+                d.push_back(a);
+                starts.push_back(tot);
+                tot += (long)nch * a.n_outer;
+            }
+            L.desc = pool_put(b, d.data(), d.size() * sizeof(AArgs<T>));
+        }
+        L.starts = (const long *)pool_put(b, starts.data(), starts.size() * sizeof(long));
+        L.nblocks = tot;
+        if (!L.desc || !L.starts) return MPC_ENOMEM;
+        out.push_back(std::move(L));
+    }
+    return 0;
+}
+
+template <class T> static int batch_launch(mpc_batch *b, const BLaunch &L) {
+    if (L.kind == 't')
+        return launch_tline_batch<T>((const TArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.plasma != 0,
+                                     b->stream);
+    return launch_axis_batch<T>((const AArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.variant, b->stream);
+}
+
+/* the launch list of a pass, as a graph key */
+static std::string launch_key(const std::vector<BLaunch> &ls, size_t i0, size_t i1) {
+    std::string k;
+    for (size_t i = i0; i < i1; i++) {
+        const BLaunch &L = ls[i];
+        char buf[128];
+        snprintf(buf, sizeof buf, "%c%d.%d.%d.%d.%ld.%p.%p;", L.kind, L.len, L.plasma, L.variant, L.n, L.nblocks,
+                 L.desc, (const void *)L.starts);
+        k += buf;
+    }
+    return k;
+}
+
+template <class T> static int batch_run(mpc_batch *b, int step0, int nsteps, bool use_graph) {
+    const size_t np = b->ctx.size();
+    std::vector<std::vector<SweepOp>> plans(np);
+    std::vector<char> fend(np);
+    for (size_t p = 0; p < np; p++) {
+        bool fe = false;
+        plan_fixed(b->ctx[p], step0, nsteps, plans[p], &fe);
+        fend[p] = fe;
+        if (plans[p].size() != plans[0].size()) return MPC_EINVAL;
+        for (size_t i = 0; i < plans[p].size(); i++)
+            if (plans[p][i].kind != plans[0][i].kind || plans[p][i].step != plans[0][i].step) return MPC_EINVAL;
+    }
+    /* every table and descriptor exists before anything is launched or captured */
+    std::vector<BLaunch> ls;
+    for (size_t i = 0; i < plans[0].size(); i++) RCHK(batch_group<T>(b, plans, i, ls));
+    for (mpc_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    size_t i0 = 0;
+    while (i0 < ls.size()) {
+        /* launches of one pass */
+        const int pass = ls[i0].step / b->S;
+        size_t i1 = i0;
+        while (i1 < ls.size() && ls[i1].step / b->S == pass) i1++;
+        if (!use_graph) {
+            for (size_t i = i0; i < i1; i++) RCHK(batch_launch<T>(b, ls[i]));
+        } else {
+            const std::string key = launch_key(ls, i0, i1);
+            auto it = b->graphs.find(key);
+            if (it == b->graphs.end()) {
+                mpc_batch_s::Graph g;
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (size_t i = i0; i < i1 && !rc; i++) rc = batch_launch<T>(b, ls[i]);
+                const hipError_t ce = hipStreamEndCapture(b->stream, &g.g);
+                if (rc) { if (g.g) (void)hipGraphDestroy(g.g); return rc; }
+                HIPCHK(ce);
+                HIPCHK(hipGraphInstantiate(&g.e, g.g, nullptr, nullptr, 0));
+                it = b->graphs.emplace(key, g).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second.e, b->stream));
+        }
+        i0 = i1;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t p = 0; p < np; p++) {
+        mpc_ctx *c = b->ctx[p];
+        c->domain_freq = fend[p] ? 1 : 0;
+        if (c->p.plasm == 1) c->rho_nvalid = (int)(c->nxy / tline_lines_per_block(c->nt, true, b->prec));
+    }
+    return 0;
+}
+
+extern "C" int mpc_batch_create(mpc_ctx *const *ctxs, int n, mpc_batch **out) {
+    if (!ctxs || n <= 0 || !out) return MPC_EINVAL;
+    *out = nullptr;
+    auto *b = new mpc_batch_s();
+    for (int i = 0; i < n; i++) {
+        mpc_ctx *c = ctxs[i];
+        if (!c) { delete b; return MPC_EINVAL; }
+        const int S = c->p.steps_per_pass > 0 ? c->p.steps_per_pass : 1;
+        if (i == 0) { b->prec = c->prec; b->S = S; }
+        if (c->prec != b->prec || S != b->S || c->p.npasses != ctxs[0]->p.npasses || c->device != ctxs[0]->device) {
+            delete b;
+            return MPC_EINVAL;
+        }
+        b->ctx.push_back(c);
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return MPC_EHIP; }
+    *out = b;
+    return 0;
+}
+
+extern "C" void mpc_batch_destroy(mpc_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &g : b->graphs) {
+        if (g.second.e) (void)hipGraphExecDestroy(g.second.e);
+        if (g.second.g) (void)hipGraphDestroy(g.second.g);
+    }
+    for (auto &d : b->pool) (void)hipFree(d.second);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int mpc_batch_run(mpc_batch *b, int step0, int nsteps, int use_graph) {
+    if (!b || nsteps < 0) return MPC_EINVAL;
+    if (nsteps == 0) return 0;
+    return b->prec == MPC_F32 ? batch_run<float>(b, step0, nsteps, use_graph != 0)
+                              : batch_run<double>(b, step0, nsteps, use_graph != 0);
+}
+
+extern "C" int mpc_batch_graphs(mpc_batch *b) { return b ? (int)b->graphs.size() : MPC_EINVAL; }
 
 extern "C" int mpc_set_timing(mpc_ctx *c, int enable) {
     if (!c) return MPC_EINVAL;

@@ -50,14 +50,13 @@ template <class T> __device__ __forceinline__ void axis_tile_of(const AArgs<T> &
  * workgroup per tile and the tables are read per tile.
  */
 template <class T, int N, class PL, int G_, bool PERSIST>
-__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T> a) {
+__device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstride) {
     using C = ACfg<T, N, PL, G_>;
     using CO = ChainOf<T, N, C::E, typename PL::R>;
     constexpr int E = C::E, TPL = C::TPL, G = C::G;
     __shared__ __align__(16) cx<T> lds[N * G];
     const int gcol0 = threadIdx.x % G, tid0 = threadIdx.x / G;
     const long ntiles = (long)(a.nt / G) * a.n_outer;
-    long lin = blockIdx.x;
     if (lin >= ntiles) return;                        /* whole workgroup */
 
     std::conditional_t<PERSIST, typename CO::type, NoChain> twc;
@@ -97,7 +96,7 @@ __global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T>
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         const cx<T> p = pn;
-        const long lnext = lin + gridDim.x;
+        const long lnext = lin + lstride;
         if constexpr (PERSIST) {
             if (lnext < ntiles) {
                 long on, tn;
@@ -125,6 +124,30 @@ __global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T>
         if (!PERSIST || lnext >= ntiles) break;
         lin = lnext;
     }
+}
+template <class T, int N, class PL, int G_, bool PERSIST>
+__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS)) k_axis(AArgs<T> a) {
+    axis_work<T, N, PL, G_, PERSIST>(a, blockIdx.x, gridDim.x);
+}
+/*
+ * Batched form: one workgroup per tile of every field in d[0..n), which
+ * share the line length; workgroup b takes field p, tile b - tstart[p], for
+ * the largest p with tstart[p] <= b.  Same bits per field as k_axis<...,
+ * false> with that field's tile order.
+ */
+template <class T, int N, class PL, int G_>
+__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS))
+k_axis_batch(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, int n) {
+    const long b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const AArgs<T> a = d[lo];
+    const long t = b - tstart[lo];
+    axis_work<T, N, PL, G_, false>(a, t, (long)(a.nt / G_) * a.n_outer);
 }
 
 bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
@@ -172,6 +195,50 @@ template <class T, int N> static int axis_v(const AArgs<T> &a, int outer, int va
     const int plan = variant % 4, order = (variant / 4) % 4;
     return (variant / 16) ? axis_p<T, N, 1>(a, outer, plan, order, s) : axis_p<T, N, 0>(a, outer, plan, order, s);
 }
+/* the tile geometry a variant gives: t values per tile, and tile order */
+int axis_tile_g(int precision, int variant) {
+    const int G = precision == MPC_F32 ? ATile<float>::G : ATile<double>::G;
+    return (variant % 4 == 1 || variant % 4 == 2) ? G / 2 : G;
+}
+int axis_tile_tg(int variant, int nchunks) { return axis_tg((variant / 4) % 4, nchunks); }
+
+template <class T, int N, class PL, int G>
+static int axis_batch_go(const AArgs<T> *d, const long *tstart, int n, long nblocks, hipStream_t s) {
+    using C = ACfg<T, N, PL, G>;
+    if (n <= 0 || nblocks <= 0 || nblocks > 0x7fffffffL) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_axis_batch<T, N, PL, G>), dim3((unsigned)nblocks), dim3(C::THREADS), 0, s, d, tstart, n);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T, int N>
+static int axis_batch_v(const AArgs<T> *d, const long *tstart, int n, long nblocks, int variant, hipStream_t s) {
+    constexpr int G = ATile<T>::G;
+    switch (variant % 4) {
+    case 1: return axis_batch_go<T, N, Plan<N>, G / 2>(d, tstart, n, nblocks, s);
+    case 2: return axis_batch_go<T, N, APlan8<N>, G / 2>(d, tstart, n, nblocks, s);
+    case 3: return axis_batch_go<T, N, APlan8<N>, G>(d, tstart, n, nblocks, s);
+    default: return axis_batch_go<T, N, Plan<N>, G>(d, tstart, n, nblocks, s);
+    }
+}
+/* batched x/y sweep: d[p] carries each field's strides, tables, n_outer and
+ * tg (as the variant's tile order gives them); the plan and tile width come
+ * from variant % 4 */
+template <class T>
+int launch_axis_batch(const AArgs<T> *d, const long *tstart, int n, long nblocks, int len, int variant, hipStream_t s) {
+    if (variant < 0 || variant >= axis_variants()) return MPC_EINVAL;
+    switch (len) {
+    case 16:  return axis_batch_v<T, 16>(d, tstart, n, nblocks, variant, s);
+    case 32:  return axis_batch_v<T, 32>(d, tstart, n, nblocks, variant, s);
+    case 64:  return axis_batch_v<T, 64>(d, tstart, n, nblocks, variant, s);
+    case 128: return axis_batch_v<T, 128>(d, tstart, n, nblocks, variant, s);
+    case 192: return axis_batch_v<T, 192>(d, tstart, n, nblocks, variant, s);
+    case 256: return axis_batch_v<T, 256>(d, tstart, n, nblocks, variant, s);
+    case 512: return axis_batch_v<T, 512>(d, tstart, n, nblocks, variant, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_axis_batch<double>(const AArgs<double> *, const long *, int, long, int, int, hipStream_t);
+template int launch_axis_batch<float>(const AArgs<float> *, const long *, int, long, int, int, hipStream_t);
+
 template <class T> int launch_axis(const AArgs<T> &a, int n, int outer, int variant, hipStream_t s) {
     if (variant < 0 || variant >= axis_variants()) return MPC_EINVAL;
     switch (n) {

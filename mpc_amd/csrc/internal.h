@@ -98,6 +98,17 @@ int launch_adk(const double *E, long n, const double *kon, double *W1, double *W
 bool tline_supported(int nt);
 bool axis_supported(int n);
 int axis_tile(int precision);   /* G: t values per axis tile */
+int axis_tile_g(int precision, int variant);   /* t values per tile of an axis variant */
+int axis_tile_tg(int variant, int nchunks);    /* tile-order run length of an axis variant */
+int tline_lines_per_block(int nt, bool plasma, int prec);
+
+/* batched sweeps over several fields of one size class (one workgroup per
+ * line group / tile of every field; descriptors and start offsets in device
+ * memory) */
+template <class T>
+int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long nblocks, int nt, bool plasma, hipStream_t s);
+template <class T>
+int launch_axis_batch(const AArgs<T> *d, const long *tstart, int n, long nblocks, int len, int variant, hipStream_t s);
 
 } // namespace mpc
 

@@ -26,11 +26,11 @@ namespace mpc {
  * behind the previous line's FFTs.  Without PERSIST the grid has one
  * workgroup per line group and every table is read per line.
  */
-/* the FP32 persistent form is held to two waves per SIMD (256 VGPRs), which
- * it meets with no more than a few spilled registers */
+/* the work of one workgroup: line groups grp, grp + gstride, ... of the
+ * field a describes (one group unless PERSIST); its plasma maximum goes to
+ * a.rho_block[rho_slot] */
 template <class T, class G, bool PLASMA, bool PERSIST>
-__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), (PERSIST && sizeof(T) == 4) ? 2 : 1)
-k_tline(TArgs<T> a) {
+__device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gstride, long rho_slot) {
     using C = TCfg<T, G, PLASMA>;
     using PL = typename G::PL;
     using CO = ChainOf<T, G::NT, C::E, typename PL::R>;
@@ -39,7 +39,6 @@ k_tline(TArgs<T> a) {
     __shared__ __align__(16) unsigned char smem[LPB * C::LINE_BYTES];
     const int lslot0 = threadIdx.x / TPL, tid0 = threadIdx.x % TPL;
     const long ngroups = a.nlines / LPB;
-    long grp = blockIdx.x;
     if (grp >= ngroups) return;                       /* whole workgroup */
     int tid = tid0, lslot = lslot0;
 
@@ -88,7 +87,7 @@ k_tline(TArgs<T> a) {
 #pragma unroll
             for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
         }
-        const long gnext = grp + gridDim.x;
+        const long gnext = grp + gstride;
         if constexpr (PERSIST) {
             if (gnext < ngroups) {
                 const cx<T> *gn = a.field + (gnext * LPB + lslot) * NT;
@@ -164,10 +163,40 @@ k_tline(TArgs<T> a) {
             if (threadIdx.x == 0) {
                 double mx = 0.0;
                 for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mx = fmax(mx, red[i]);
-                a.rho_block[blockIdx.x] = mx;
+                a.rho_block[rho_slot] = mx;
             }
         }
     }
+}
+
+/* the FP32 persistent form is held to two waves per SIMD (256 VGPRs), which
+ * it meets with no more than a few spilled registers */
+template <class T, class G, bool PLASMA, bool PERSIST>
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), (PERSIST && sizeof(T) == 4) ? 2 : 1)
+k_tline(TArgs<T> a) {
+    tline_work<T, G, PLASMA, PERSIST>(a, blockIdx.x, gridDim.x, blockIdx.x);
+}
+
+/*
+ * Batched form: one workgroup per line group of every field in d[0..n),
+ * which share the t length and the plasma switch.  Workgroup b works on
+ * field p, group b - gstart[p], for the largest p with gstart[p] <= b; the
+ * per-line arithmetic is the per-field kernel's, so each field's result is
+ * the same bits as a launch of k_tline<..., false> on it alone.
+ */
+template <class T, class G, bool PLASMA>
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS))
+k_tline_batch(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart, int n) {
+    const long b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (gstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TArgs<T> a = d[lo];
+    const long g = b - gstart[lo];
+    tline_work<T, G, PLASMA, false>(a, g, a.nlines, g);
 }
 
 bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
@@ -197,6 +226,31 @@ template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, in
         return plasma ? tline_go<T, GEO, true, true>(a, s, nb) : tline_go<T, GEO, false, true>(a, s, nb);
     return plasma ? tline_go<T, GEO, true, false>(a, s, nb) : tline_go<T, GEO, false, false>(a, s, nb);
 }
+template <class T, int NT>
+static int tline_batch_go(const TArgs<T> *d, const long *gstart, int n, long nblocks, bool plasma, hipStream_t s) {
+    using GEO = TGeo<NT>;
+    if (n <= 0 || nblocks <= 0 || nblocks > 0x7fffffffL) return MPC_EINVAL;
+    if (plasma)
+        hipLaunchKernelGGL((k_tline_batch<T, GEO, true>), dim3((unsigned)nblocks), dim3(TCfg<T, GEO, true>::THREADS),
+                           0, s, d, gstart, n);
+    else
+        hipLaunchKernelGGL((k_tline_batch<T, GEO, false>), dim3((unsigned)nblocks), dim3(TCfg<T, GEO, false>::THREADS),
+                           0, s, d, gstart, n);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T>
+int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long nblocks, int nt, bool plasma, hipStream_t s) {
+    switch (nt) {
+    case 512:  return tline_batch_go<T, 512>(d, gstart, n, nblocks, plasma, s);
+    case 1024: return tline_batch_go<T, 1024>(d, gstart, n, nblocks, plasma, s);
+    case 2048: return tline_batch_go<T, 2048>(d, gstart, n, nblocks, plasma, s);
+    case 4096: return tline_batch_go<T, 4096>(d, gstart, n, nblocks, plasma, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_tline_batch<double>(const TArgs<double> *, const long *, int, long, int, bool, hipStream_t);
+template int launch_tline_batch<float>(const TArgs<float> *, const long *, int, long, int, bool, hipStream_t);
+
 template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks) {
     if (variant < 0 || variant >= tline_variants()) return MPC_EINVAL;
     switch (nt) {

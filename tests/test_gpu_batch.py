@@ -1,0 +1,121 @@
+"""GPU: batched runs (many pulses, one launch per sweep and size class, a HIP
+graph per pass) against per-pulse runs and the oracle.
+
+Bar: each pulse's batched result is bit-identical to mpc_run on its own
+context with the one-workgroup-per-tile kernels the batch runs (t-line
+variant 1, the same x/y plan); one pulse is also checked against the FP64
+oracle at 1e-10.  (The persistent variants do the same arithmetic per line
+but are compiled separately and need not round identically; the variant test
+checks them against the oracle.)
+Graph replays are interleaved with host downloads and eager work, and every
+replay's result is compared.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mpc_amd import Batch, Propagator, configs, make_inputs
+
+pytestmark = pytest.mark.gpu
+
+S, PASSES = 3, 3
+# uneven grids, energies, pressures and waists; two pulses share a size class
+SPECS = [dict(nx=32, nt=512, energy=1e-3, pressure=2.0, seed=11),
+         dict(nx=64, nt=512, energy=3e-4, pressure=1.0, seed=12, waist=0.3e-3),
+         dict(nx=32, nt=1024, energy=2e-4, pressure=2.5, seed=13),
+         dict(nx=32, nt=512, energy=2e-4, pressure=0.5, seed=14, waist=0.25e-3)]
+
+
+def _inputs(mirror_R=0.3):
+    out = []
+    for s in SPECS:
+        inp = make_inputs(configs.small(steps_per_pass=S, passes=PASSES, mirror_R=mirror_R, **s))
+        out.append((inp, inp.calibrated()))
+    return out
+
+
+def _props(cases, prec, variants):
+    props = []
+    for inp, A0 in cases:
+        p = Propagator(inp, prec)
+        p.upload(A0)
+        p.set_variants(*variants)
+        props.append(p)
+    return props
+
+
+def _close(props):
+    for p in props:
+        p.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("mirror_R", [0.3, 0.0])
+def test_batch_is_bitwise_per_pulse(gpu_lib, prec, mirror_R):
+    """one call over all passes, with and without graphs, against per-pulse runs"""
+    nsteps = S * PASSES
+    cases = _inputs(mirror_R)
+    print(f"case batch one-shot prec={prec} mirror_R={mirror_R}", flush=True)
+    # per-pulse runs: per-line-group t-line, x/y plans 2 and 3
+    ref = []
+    for p in _props(cases, prec, (1, 2, 3)):
+        p.run(0, nsteps)
+        ref.append(p.download())
+        assert np.isfinite(ref[-1]).all()
+        d = p.diagnostics()
+        p.close()
+    for graph in (False, True):
+        props = _props(cases, prec, (0, 18, 19))    # the batch takes plan and order, not mode
+        with Batch(props) as b:
+            b.run(0, nsteps, graph=graph)
+            for i, p in enumerate(props):
+                got = p.download()
+                assert np.array_equal(got, ref[i]), (prec, mirror_R, graph, i)
+            if graph:
+                print(f"  graphs captured: {b.graphs()}", flush=True)
+                assert 1 <= b.graphs() <= PASSES
+        # the batch leaves each context usable: diagnostics read its last step
+        dd = props[-1].diagnostics()
+        assert dd["rho_peak"] == d["rho_peak"]
+        _close(props)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-4)])
+def test_batch_pass_by_pass_graph_replays(gpu_lib, prec, tol):
+    """one call per pass: interior passes replay the same graph; every replay
+    is compared with per-pulse runs split the same way, and pulse 0 with the
+    oracle"""
+    cases = _inputs(0.3)
+    print(f"case batch pass-by-pass prec={prec}", flush=True)
+    ref_props = _props(cases, prec, (1, 2, 3))
+    props = _props(cases, prec, (0, 2, 3))
+    oracle_A = cases[0][1]
+    with Batch(props) as b:
+        for k in range(PASSES):
+            for p in ref_props:
+                p.run(k * S, S)
+            b.run(k * S, S, graph=True)
+            oracle_A, _ = O.run_fixed(cases[0][0], oracle_A, k * S, S)
+            for i, p in enumerate(props):
+                got = p.download()          # host work between replays
+                assert np.array_equal(got, ref_props[i].download()), (prec, k, i)
+            err = O.rel_l2(props[0].download(), oracle_A)
+            print(f"  pass {k}: graphs={b.graphs()} pulse-0 rel_l2 vs oracle {err:.3e}", flush=True)
+            assert err <= tol
+        # passes 1 and 2 have the same launch list as pass 0 except the last
+        # (no mirror after it): at most two graphs
+        assert b.graphs() <= 2
+    _close(props)
+    _close(ref_props)
+
+
+def test_batch_rejects_mixed_schedules(gpu_lib):
+    a = make_inputs(configs.small(32, 512, steps_per_pass=3, passes=3))
+    c = make_inputs(configs.small(32, 512, steps_per_pass=4, passes=3))
+    p1, p2 = Propagator(a, "f32"), Propagator(c, "f32")
+    with pytest.raises(RuntimeError):
+        Batch([p1, p2])
+    p3 = Propagator(a, "f64")
+    with pytest.raises(RuntimeError):
+        Batch([p1, p3])
+    _close([p1, p2, p3])
