@@ -109,6 +109,60 @@ def cpu_baseline(inp, max_seconds=30.0):
                       f"FP64 oracle (oracle/, OpenMP {threads} threads), {dt:.1f} s"}
 
 
+def batch_bench(steps, warmup):
+    """the 48-pulse batched workload in FP32: every pulse advances `steps`
+    z-steps in one batched call (one launch per sweep and size class, a HIP
+    graph per pass), against the same pulses run one after another.  The rate
+    is in 256x256x2048-equivalent steps/s: sum_p N_p * steps / N_ref per s.
+    `steps` must stay below the 300 steps of a pass (no mirror inside)."""
+    from mpc_amd import Batch, Propagator, configs, make_inputs
+    wls = configs.cfg4_batch()
+    props, npts = [], []
+    tuned = {}
+    t0 = time.perf_counter()
+    for wl in wls:
+        inp = make_inputs(wl)
+        p = Propagator(inp, "f32")
+        p.upload(inp.A)
+        p.calibrate()
+        inp.A = None                             # host copy no longer needed
+        key = (wl.nx, wl.nt)
+        if key not in tuned:
+            tuned[key] = p.autotune()
+        else:
+            p.set_variants(*tuned[key])
+        props.append(p)
+        npts.append(wl.nx * wl.nx * wl.nt)
+    log(f"[bench] batch: {len(props)} pulses set up in {time.perf_counter() - t0:.1f}s; "
+        f"grids {sorted(set((w.nx, w.nt) for w in wls))}; variants {tuned}")
+    nref = 256 * 256 * 2048
+    work = sum(npts) * steps / nref
+    b = Batch(props)
+    # the warm-up call has the timed call's launch list (same length, no
+    # mirror inside), so the pass graph is captured before the timed region
+    b.run(0, steps, graph=True)
+    t0 = time.perf_counter()
+    b.run(steps, steps, graph=True)
+    dt_b = time.perf_counter() - t0
+    log(f"[bench] batch: {steps} steps of all pulses in {dt_b:.3f}s ({work / dt_b:.1f} equivalent steps/s), "
+        f"graphs {b.graphs()}")
+    b.close()
+    t0 = time.perf_counter()
+    for p in props:
+        p.run(2 * steps, steps)
+    dt_s = time.perf_counter() - t0
+    log(f"[bench] one by one: {dt_s:.3f}s ({work / dt_s:.1f} equivalent steps/s)")
+    for p in props:
+        p.close()
+    return {"metric": "batched split-step, 256x256x2048-equivalent steps/s (48 pulses, FP32)",
+            "value": round(work / dt_b, 3), "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded pulses as drawn below)",
+            "config": {"workload": "48 pulses drawn from {128^2x512, 192^2x1024, 256^2x2048}, 0.1-2 mJ, "
+                                   "100-500 fs, Ar 0.5-3 bar, w0 0.5-1 mm; 10 passes x 300 steps",
+                       "points_total": int(sum(npts)), "one_by_one_steps_per_s": round(work / dt_s, 3),
+                       "graphs": True}}
+
+
 def load_traffic(prec):
     path = os.path.join(ROOT, "profiles", f"traffic_tline_256x256x2048_{prec}.json")
     if not os.path.exists(path):
@@ -124,6 +178,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
+    ap.add_argument("--batch", action="store_true",
+                    help="instead: the 48-pulse batched workload (FP32), batched vs one by one")
     ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
                     help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
@@ -138,6 +194,9 @@ def main(argv=None):
     if lib().mpc_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if a.batch:
+        print(json.dumps(batch_bench(a.steps, a.warmup)))
+        return 0
     wl = configs.cfg2_production_256()
     wl.nx, wl.nt = a.nx, a.nt
     t0 = time.perf_counter()

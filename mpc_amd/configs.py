@@ -62,6 +62,26 @@ def cfg3_precision_64() -> Workload:
     return w
 
 
+def cfg4_batch(npulses: int = 48, seed: int = 20261202 + 3) -> list:
+    """48 pulses with grids drawn by seed from {128^2 x 512, 192^2 x 1024,
+    256^2 x 2048}, energies log-uniform 0.1-2 mJ, durations uniform 100-500
+    fs, argon 0.5-3 bar, w0 0.5-1.0 mm; 10 passes x 300 steps.  Every grid
+    spans the 6 mm x 4 ps window of the 256^2 x 2048 workload."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    grids = [(128, 512), (192, 1024), (256, 2048)]
+    out = []
+    for i in range(npulses):
+        nx, nt = grids[int(rng.integers(len(grids)))]
+        energy = float(10 ** rng.uniform(np.log10(0.1e-3), np.log10(2e-3)))
+        tau = float(rng.uniform(100, 500)) * FS
+        pressure = float(rng.uniform(0.5, 3.0))
+        waist = float(rng.uniform(0.5e-3, 1.0e-3))
+        out.append(Workload(f"batch pulse {i}: {nx}x{nx}x{nt}", nx, nt, 6e-3, 4e-12, waist, tau, energy, pressure,
+                            40.0, 10, 300, seed=seed * 100 + i))
+    return out
+
+
 def small(nx: int = 16, nt: int = 64, plasm: int = 1, sst_on: int = 1, mirror_R: float = 0.3,
           energy: float = 1e-3, steps_per_pass: int = 4, passes: int = 2, seed: int = 7,
           pressure: float = 2.0, waist: float = 0.2e-3) -> Workload:
