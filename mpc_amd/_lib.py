@@ -101,6 +101,22 @@ def lib() -> C.CDLL:
         "mpc_batch_destroy": (None, [vp]),
         "mpc_batch_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpc_batch_graphs": (C.c_int, [vp]),
+        "mpc_sizeof_gemm": (C.c_int, []),
+        "mpc_gemm_bf16": (C.c_int, [vp, vp]),
+        "mpc_fno_combine": (C.c_int, [vp, vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+        "mpc_fno_bias_act": (C.c_int, [vp, vp, vp, C.c_long, C.c_int, C.c_long, C.c_int, vp]),
+        "mpc_fno_act_bwd": (C.c_int, [vp, vp, vp, C.c_long, C.c_int, vp]),
+        "mpc_fno_bias_grad": (C.c_int, [vp, vp, C.c_long, C.c_int, C.c_long, C.c_int, vp]),
+        "mpc_fno_adam": (C.c_int, [vp, vp, vp, vp, C.c_long, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, vp]),
+        "mpc_fno_rel_l2": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_long, vp]),
+        "mpc_dev_alloc": (C.c_int, [C.c_long, C.POINTER(C.c_void_p)]),
+        "mpc_dev_free": (None, [vp]),
+        "mpc_dev_upload": (C.c_int, [vp, vp, C.c_long]),
+        "mpc_dev_download": (C.c_int, [vp, vp, C.c_long]),
+        "mpc_dev_copy": (C.c_int, [vp, vp, C.c_long, vp]),
+        "mpc_dev_fill": (C.c_int, [vp, C.c_long, C.c_float, vp]),
+        "mpc_dev_sync": (C.c_int, []),
         "mpc_mkinput": (C.c_int, [C.POINTER(MpcInputSpec), C.POINTER(MpcParams), dptr, dptr, dptr, dptr, vp]),
         "mpc_write_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),
         "mpc_read_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),

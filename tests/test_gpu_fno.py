@@ -1,0 +1,145 @@
+"""GPU: the FNO surrogate's kernels and network.
+
+* mpc_gemm_bf16 against the FP64 product of the BF16-rounded operands, over
+  transposed operands, two-level batches, alpha/beta and split K (FP32
+  accumulation: 1e-4 of the result's norm).
+* k_combine forward against numpy and its bwd map as the exact transpose.
+* the network's forward against the FP64 CPU network (fno_cpu) to BF16
+  tolerance (3e-2), and the hand-written backward's gradient of every
+  parameter tensor against central finite differences of the CPU loss along
+  a random direction (5e-2 of the larger of the difference quotient and the
+  gradient's norm).
+* Adam on the GPU lowers the loss of a learnable map.
+"""
+import numpy as np
+import pytest
+
+import fno_cpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(a):
+    """round float32 values to BF16 (nearest even), kept as float64"""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _t(a):
+    from mpc_amd.fno import DArr
+    return DArr.of(a)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 32, False, False), (70, 33, 45, True, False),
+                                         (17, 130, 64, False, True), (100, 90, 200, True, True),
+                                         (32, 32, 8192, False, True)])
+def test_gemm_matches_bf16_product(gpu_lib, M, N, K, ta, tb):
+    from mpc_amd.fno import Ops
+    print(f"case gemm M={M} N={N} K={K} ta={ta} tb={tb}", flush=True)
+    rng = np.random.default_rng(M * 7 + N + K)
+    nb1, nb2 = (2, 3) if K < 1000 else (1, 1)
+    A = rng.standard_normal((nb1, nb2, M, K)).astype(np.float32)
+    B = rng.standard_normal((nb1, nb2, K, N)).astype(np.float32)
+    C0 = rng.standard_normal((nb1, nb2, M, N)).astype(np.float32)
+    alpha, beta = 0.75, 0.5
+    ref = alpha * np.einsum("abik,abkj->abij", _bf16(A), _bf16(B)) + beta * C0
+    As = np.swapaxes(A, 2, 3).copy() if ta else A       # stored transposed: element (i,k) at k*M + i
+    Bs = np.swapaxes(B, 2, 3).copy() if tb else B
+    dA, dB, dC = _t(As), _t(Bs), _t(C0)
+    rsA, csA = (1, M) if ta else (K, 1)
+    rsB, csB = (1, K) if tb else (N, 1)
+    ops = Ops()
+    ops.gemm(M, N, K, dA, rsA, csA, dB, rsB, csB, dC, N, 1, alpha=alpha, beta=beta,
+             b1=(nb1, nb2 * M * K, nb2 * K * N, nb2 * M * N), b2=(nb2, M * K, K * N, M * N))
+    got = dC.numpy().astype(np.float64)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err <= 1e-4
+
+
+def test_combine_and_its_transpose(gpu_lib):
+    from mpc_amd.fno import DArr, Ops
+    rng = np.random.default_rng(5)
+    ns, m, nb, h = 3, 4, 5, 6
+    P = rng.standard_normal((ns, 2 * m, nb, 2 * h)).astype(np.float32)
+    dV = rng.standard_normal((ns, m, nb, 2 * h)).astype(np.float32)
+    ops = Ops()
+    for sign in (-1, 1):
+        V = DArr((ns, m, nb, 2 * h))
+        ops.combine(_t(P), V, ns, m, nb, h, sign)
+        dP = DArr((ns, 2 * m, nb, 2 * h))
+        ops.combine(_t(dV), dP, ns, m, nb, h, sign, bwd=True)
+        c, s = P[:, :m], P[:, m:]
+        ref = np.concatenate([c[..., :h] - sign * s[..., h:], c[..., h:] + sign * s[..., :h]], axis=-1)
+        assert np.allclose(V.numpy(), ref, atol=1e-6)
+        lhs = float(np.sum(V.numpy().astype(np.float64) * dV))
+        rhs = float(np.sum(P.astype(np.float64) * dP.numpy()))
+        assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def _small():
+    from mpc_amd.fno import FnoConfig, init_params
+    cfg = FnoConfig(X=16, Y=8, T=32, width=8, layers=2, mx=4, my=4, mt=6, hidden=16)
+    p = init_params(cfg, seed=4)
+    for l in range(cfg.layers):        # spectral weights of a size that matters next to the pointwise path
+        p[f"Rr{l}"] *= 20
+        p[f"Ri{l}"] *= 20
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((2, cfg.cin, cfg.X, cfg.Y, cfg.T))
+    tgt = 0.8 * x[:, :2] + 0.1 * np.roll(x[:, :2], 1, axis=-1)
+    return cfg, p, x, tgt
+
+
+def test_forward_matches_cpu_network(gpu_lib):
+    from mpc_amd.fno import FnoGpu
+    cfg, p, x, _ = _small()
+    net = FnoGpu(cfg, p)
+    got = net.forward(_t(x)).numpy()
+    ref = fno_cpu.forward(p, cfg, x)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref - x[:, :2])
+    print(f"case fno forward: rel err of the change {err:.2e}", flush=True)
+    assert err <= 3e-2
+
+
+def test_backward_matches_finite_differences(gpu_lib):
+    from mpc_amd.fno import FnoGpu
+    cfg, p, x, tgt = _small()
+    net = FnoGpu(cfg, p)
+    loss_gpu = net.loss_and_grad(_t(x), _t(tgt))
+    grads = net.grads()
+    loss_cpu = fno_cpu.loss(p, cfg, x, tgt)
+    print(f"case fno gradients: loss gpu {loss_gpu:.6f} cpu {loss_cpu:.6f}", flush=True)
+    assert abs(loss_gpu - loss_cpu) <= 2e-2 * loss_cpu
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for k in p:
+        d = rng.standard_normal(p[k].shape)
+        eps = 1e-4 * max(1.0, float(np.abs(p[k]).max())) / max(1.0, float(np.abs(d).max()))
+        pp = {kk: vv.copy() for kk, vv in p.items()}
+        pm = {kk: vv.copy() for kk, vv in p.items()}
+        pp[k] = p[k] + eps * d
+        pm[k] = p[k] - eps * d
+        fd = (fno_cpu.loss(pp, cfg, x, tgt) - fno_cpu.loss(pm, cfg, x, tgt)) / (2 * eps)
+        an = float(np.sum(grads[k] * d))
+        # measured against the larger of |fd| and |g_k|, the rms of <g_k, d>
+        # over random directions d: a small projection is held to the
+        # gradient's own scale, not to its accidental smallness
+        rel = abs(an - fd) / max(abs(fd), float(np.linalg.norm(grads[k])), 1e-12)
+        worst = max(worst, rel)
+        print(f"  {k:4s} fd {fd:+.6e} gpu {an:+.6e} rel {rel:.2e}", flush=True)
+        assert rel <= 5e-2, k
+    print(f"  worst {worst:.2e}", flush=True)
+
+
+def test_adam_training_lowers_the_loss(gpu_lib):
+    from mpc_amd.fno import FnoGpu
+    cfg, p, x, tgt = _small()
+    net = FnoGpu(cfg, p)
+    X, T = _t(x), _t(tgt)
+    first = net.loss_and_grad(X, T)
+    for _ in range(60):
+        net.adam_step(lr=3e-3)
+        last = net.loss_and_grad(X, T)
+    print(f"case fno adam: loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last <= 0.5 * first
