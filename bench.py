@@ -2,6 +2,8 @@
 """Benchmark: split-step steps/s of the 256x256x2048 argon workload on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py --batch        (the 48-pulse batched workload instead)
+    python bench.py --surrogate    (the pass surrogate experiment instead)
 
 One "step" is one z-step L(dz/2) N(dz) L(dz/2) of the whole 256x256x2048 field
 (Kerr + self-steepening + ADK/avalanche plasma + diffraction + dispersion),
@@ -180,6 +182,12 @@ def main(argv=None):
     ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
     ap.add_argument("--batch", action="store_true",
                     help="instead: the 48-pulse batched workload (FP32), batched vs one by one")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="instead: the pass-surrogate experiment (pairs from the solver, training on a time "
+                         "budget, held-out errors, assisted rollouts against plain ones)")
+    ap.add_argument("--surrogate-pulses", type=int, default=512, help=argparse.SUPPRESS)
+    ap.add_argument("--surrogate-passes", type=int, default=8, help=argparse.SUPPRESS)
+    ap.add_argument("--train-seconds", type=float, default=240.0, help=argparse.SUPPRESS)
     ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
                     help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
@@ -196,6 +204,17 @@ def main(argv=None):
         return 1
     if a.batch:
         print(json.dumps(batch_bench(a.steps, a.warmup)))
+        return 0
+    if a.surrogate:
+        from mpc_amd import surrogate
+        r = surrogate.run_experiment(n_pulses=a.surrogate_pulses, passes=a.surrogate_passes,
+                                     train_seconds=a.train_seconds, log=log)
+        print(json.dumps({"metric": "surrogate-assisted passes/s at 64x64x256 (rel L2 per pass vs plain FP32)",
+                          "value": r["passes_per_s_assisted"], "unit": "passes/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "bf16 (network), f32 (solver)",
+                          "data": "synthetic pulses drawn by seed", "vs_baseline": None,
+                          "config": {"workload": "pass surrogate, 64x64x256, 200 z-steps per pass"},
+                          "result": r}))
         return 0
     wl = configs.cfg2_production_256()
     wl.nx, wl.nt = a.nx, a.nt

@@ -93,7 +93,19 @@ int  mpc_run(mpc_ctx *ctx, int step0, int nsteps);
 int  mpc_step(mpc_ctx *ctx, double dzz, double z, int *signmem);
 /* mirror (curvature x aperture x sqrt(Ab_M)) applied once to the field */
 int  mpc_mirror(mpc_ctx *ctx);
+/* Linear propagation only, over `length` (m) in one go: dispersion and
+ * absorption on the t-lines, diffraction with coefficient sign * length on x
+ * and y, then the mirror if `mirror` != 0.  Equals the fixed-step schedule of
+ * mpc_run with the nonlinear step removed (its linear factors commute and
+ * merge).  Time-domain field in and out.  No reference counterpart: the pass
+ * surrogate takes it as the physics prior that its network corrects. */
+int  mpc_linear(mpc_ctx *ctx, double length, int sign, int mirror);
 int  mpc_diagnostics(mpc_ctx *ctx, mpc_diag *d);
+/* the resident time-domain field as two float planes (re then im, N points
+ * each, point order [y][x][t], t fastest) in a device buffer, times scale;
+ * and the reverse (the pass surrogate hands its prediction back this way) */
+int  mpc_export_planar(mpc_ctx *ctx, float *planes, double scale);
+int  mpc_import_planar(mpc_ctx *ctx, const float *planes, double scale);
 /* adaptive step doubling + Richardson (reference MPC.c:343-383,
  * propagation.c:616-677) for at most max_steps accepted steps.  z, dzz and
  * signmem are updated; errs/dzs (may be NULL) receive per accepted step. */

@@ -150,6 +150,10 @@ class Propagator:
     def mirror(self) -> None:
         check(lib().mpc_mirror(self.ctx), "mpc_mirror")
 
+    def linear(self, length: float, sign: int = 1, mirror: bool = False) -> None:
+        """linear propagation only over `length` (then the mirror if asked)"""
+        check(lib().mpc_linear(self.ctx, length, sign, 1 if mirror else 0), "mpc_linear")
+
     def diagnostics(self) -> dict:
         d = MpcDiag()
         temp, spec = np.zeros(self.nt), np.zeros(self.nt)

@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
         "mpc_run": (C.c_int, [vp, C.c_int, C.c_int]),
         "mpc_step": (C.c_int, [vp, C.c_double, C.c_double, iptr]),
         "mpc_mirror": (C.c_int, [vp]),
+        "mpc_linear": (C.c_int, [vp, C.c_double, C.c_int, C.c_int]),
         "mpc_diagnostics": (C.c_int, [vp, C.POINTER(MpcDiag)]),
         "mpc_run_adaptive": (C.c_int, [vp, dptr, dptr, iptr, C.c_int, iptr, dptr, dptr]),
         "mpc_set_timing": (C.c_int, [vp, C.c_int]),
@@ -117,6 +118,8 @@ def lib() -> C.CDLL:
         "mpc_dev_copy": (C.c_int, [vp, vp, C.c_long, vp]),
         "mpc_dev_fill": (C.c_int, [vp, C.c_long, C.c_float, vp]),
         "mpc_dev_sync": (C.c_int, []),
+        "mpc_export_planar": (C.c_int, [vp, vp, C.c_double]),
+        "mpc_import_planar": (C.c_int, [vp, vp, C.c_double]),
         "mpc_mkinput": (C.c_int, [C.POINTER(MpcInputSpec), C.POINTER(MpcParams), dptr, dptr, dptr, dptr, vp]),
         "mpc_write_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),
         "mpc_read_param_bin": (C.c_int, [C.c_char_p, C.POINTER(MpcParams)]),

@@ -761,6 +761,25 @@ extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
     return finish(c, rc);
 }
 
+/* the linear part of propagation over `length` in one go (dispersion on the
+ * t-lines, diffraction with coefficient sign * length on x and y, then the
+ * mirror if asked): the split step with the nonlinear step left out, whose
+ * linear factors commute and so merge into one table each */
+template <class T> static int linear_once(mpc_ctx *c, double length, int sign, bool mirror) {
+    const cx<T> *P = get_table<T>(c, 't', length);
+    if (!P) return MPC_ENOMEM;
+    RCHK(sweep_t<T>(c, P, nullptr, false, false, true, length));
+    return run_xy<T>(c, sign * length, mirror);
+}
+
+extern "C" int mpc_linear(mpc_ctx *c, double length, int sign, int mirror) {
+    if (!c || !(length >= 0.0) || (sign != 1 && sign != -1)) return MPC_EINVAL;
+    if (c->domain_freq) return MPC_EINVAL;
+    const int rc = c->prec == MPC_F32 ? linear_once<float>(c, length, sign, mirror != 0)
+                                      : linear_once<double>(c, length, sign, mirror != 0);
+    return finish(c, rc);
+}
+
 extern "C" int mpc_mirror(mpc_ctx *c) {
     if (!c) return MPC_EINVAL;
     /* x/y sweeps with identity propagators, mirror applied in the y sweep */
@@ -808,6 +827,26 @@ extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
     RCHK(rc);
     HIPCHK(hipMemcpyAsync(A, c->staging, 16 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
     return finish(c, 0);
+}
+
+/* the resident (time-domain) field as two float planes on the device, in the
+ * internal point order [y][x][t], times scale; and back */
+extern "C" int mpc_export_planar(mpc_ctx *c, float *planes, double scale) {
+    if (!c || !planes) return MPC_EINVAL;
+    if (c->domain_freq) return MPC_EINVAL;
+    const double s = scale * c->ascale;
+    const int rc = c->prec == MPC_F32 ? launch_planar<float>((cx<float> *)c->field, planes, c->N, s, 1, c->stream)
+                                      : launch_planar<double>((cx<double> *)c->field, planes, c->N, s, 1, c->stream);
+    return finish(c, rc);
+}
+extern "C" int mpc_import_planar(mpc_ctx *c, const float *planes, double scale) {
+    if (!c || !planes || !(scale != 0.0)) return MPC_EINVAL;
+    const double s = 1.0 / (scale * c->ascale);
+    const int rc = c->prec == MPC_F32
+        ? launch_planar<float>((cx<float> *)c->field, (float *)planes, c->N, s, 0, c->stream)
+        : launch_planar<double>((cx<double> *)c->field, (float *)planes, c->N, s, 0, c->stream);
+    c->domain_freq = 0;
+    return finish(c, rc);
 }
 
 extern "C" int mpc_calibrate(mpc_ctx *c) {

@@ -320,6 +320,7 @@ template <> struct TPlan<4096> { static constexpr int E = 8; using R = Radices<8
 template <> struct TPlan<2048> { static constexpr int E = 8; using R = Radices<8, 8, 8, 4>; };
 template <> struct TPlan<1024> { static constexpr int E = 8; using R = Radices<8, 8, 8, 2>; };
 template <> struct TPlan<512>  { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
+template <> struct TPlan<256>  { static constexpr int E = 8; using R = Radices<8, 8, 4>; };
 
 template <int N> struct Plan;
 /* t lines */

@@ -122,19 +122,34 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(mpc_gemm g) {
 }
 
 /* C = alpha * sum over splits of W + beta * C */
-__global__ void k_gemm_reduce(mpc_gemm g) {
+/* split-K reduction: workgroup of 64 outputs x 16 partial-sum lanes; lane r
+ * adds splits r, r + 16, ... and the 16 lane sums are added in lane order */
+constexpr int RD_OUT = 64, RD_SUB = 16;
+__global__ void __launch_bounds__(RD_OUT * RD_SUB) k_gemm_reduce(mpc_gemm g) {
+    __shared__ float part[RD_SUB][RD_OUT];
     const int nbat = g.nb1 * g.nb2;
     const long mn = (long)g.M * g.N;
     const long total = mn * nbat;
-    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x) {
+    const int o = threadIdx.x % RD_OUT, r = threadIdx.x / RD_OUT;
+    const long x = blockIdx.x * (long)RD_OUT + o;
+    float s = 0.f;
+    if (x < total) {
+        const int bat = (int)(x / mn);
+        const long ij = x % mn;
+        for (int sp = r; sp < g.ksplit; sp += RD_SUB) s += g.W[((long)sp * nbat + bat) * mn + ij];
+    }
+    part[r][o] = s;
+    __syncthreads();
+    if (r == 0 && x < total) {
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < RD_SUB; ++q) t += part[q][o];
         const int bat = (int)(x / mn);
         const long ij = x % mn;
         const int i = (int)(ij / g.N), j = (int)(ij % g.N);
-        float s = 0.f;
-        for (int sp = 0; sp < g.ksplit; ++sp) s += g.W[((long)sp * nbat + bat) * mn + ij];
         const int b1 = bat / g.nb2, b2 = bat % g.nb2;
         float *c = g.C + b1 * g.sC1 + b2 * g.sC2 + (long)i * g.rsC + (long)j * g.csC;
-        *c = g.beta == 0.f ? g.alpha * s : g.alpha * s + g.beta * *c;
+        *c = g.beta == 0.f ? g.alpha * t : g.alpha * t + g.beta * *c;
     }
 }
 
@@ -200,21 +215,36 @@ __global__ void k_act_bwd(const float *__restrict__ z, const float *__restrict__
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
         dz[x] = act ? dout[x] * gelu_d(z[x]) : dout[x];
 }
-/* db[c] (+)= sum over b, p of d[b][c][p]; one workgroup per channel */
-__global__ void __launch_bounds__(256) k_bias_grad(const float *__restrict__ d, float *__restrict__ db, long nb, int nc,
-                                                   long np, int accumulate) {
+/* db[c] (+)= sum over b, p of d[b][c][p], in two stages: workgroup (c, j)
+ * sums a fixed chunk of channel c into part[c][j]; then one thread per
+ * channel adds its chunks in order (deterministic) */
+constexpr int BG_CHUNKS = 64;
+__global__ void __launch_bounds__(256) k_bias_part(const float *__restrict__ d, float *__restrict__ part, long nb,
+                                                   int nc, long np) {
     __shared__ float red[256];
-    const int c = blockIdx.x;
+    const int c = blockIdx.x, j = blockIdx.y;
+    const long tot = nb * np;
+    const long per = (tot + BG_CHUNKS - 1) / BG_CHUNKS;
+    const long e0 = j * per, e1 = min(tot, e0 + per);
     float s = 0.f;
-    for (long b = 0; b < nb; ++b)
-        for (long p = threadIdx.x; p < np; p += blockDim.x) s += d[(b * nc + c) * np + p];
+    for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const long b = e / np, p = e % np;
+        s += d[(b * nc + c) * np + p];
+    }
     red[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = accumulate ? db[c] + red[0] : red[0];
+    if (threadIdx.x == 0) part[c * BG_CHUNKS + j] = red[0];
+}
+__global__ void k_bias_sum(const float *__restrict__ part, float *__restrict__ db, int nc, int accumulate) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    float s = 0.f;
+    for (int j = 0; j < BG_CHUNKS; ++j) s += part[c * BG_CHUNKS + j];
+    db[c] = accumulate ? db[c] + s : s;
 }
 /* Adam with bias correction: m, v moments; lr already corrected by the caller */
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
@@ -230,16 +260,21 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
 }
 
 /*
- * relative-L2 loss: per sample b, s[b] = (|pred-tgt|^2, |tgt|^2) (one
- * workgroup per sample), then loss[0] = mean_b s0/s1 and
- * dpred = (2/nb) (pred - tgt) / s1[b].  loss holds 1 + 2 nb floats.
+ * relative-L2 loss: per sample b, s[b] = (|pred-tgt|^2, |tgt|^2) summed in
+ * two stages (workgroup (b, j) sums chunk j into part, then in chunk order),
+ * then loss[0] = mean_b s0/s1 and dpred = (2/nb) (pred - tgt) / s1[b].
+ * loss holds 1 + 2 nb floats.
  */
-__global__ void __launch_bounds__(256) k_rel_l2_sums(const float *__restrict__ pred, const float *__restrict__ tgt,
-                                                     float *__restrict__ loss, long nper) {
+constexpr int RL_CHUNKS = 64;
+__global__ void __launch_bounds__(256) k_rel_l2_part(const float *__restrict__ pred, const float *__restrict__ tgt,
+                                                     float *__restrict__ part, long nper) {
     __shared__ float r0[256], r1[256];
     const long b = blockIdx.x;
+    const int j = blockIdx.y;
+    const long per = (nper + RL_CHUNKS - 1) / RL_CHUNKS;
+    const long e0 = j * per, e1 = min(nper, e0 + per);
     float a = 0.f, c = 0.f;
-    for (long i = threadIdx.x; i < nper; i += blockDim.x) {
+    for (long i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
         const float p = pred[b * nper + i], t = tgt[b * nper + i];
         a += (p - t) * (p - t);
         c += t * t;
@@ -251,7 +286,15 @@ __global__ void __launch_bounds__(256) k_rel_l2_sums(const float *__restrict__ p
         if ((int)threadIdx.x < o) { r0[threadIdx.x] += r0[threadIdx.x + o]; r1[threadIdx.x] += r1[threadIdx.x + o]; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { loss[1 + 2 * b] = r0[0]; loss[2 + 2 * b] = r1[0]; }
+    if (threadIdx.x == 0) { part[2 * (b * RL_CHUNKS + j)] = r0[0]; part[2 * (b * RL_CHUNKS + j) + 1] = r1[0]; }
+}
+__global__ void k_rel_l2_sums(const float *__restrict__ part, float *__restrict__ loss, int nb) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float a = 0.f, c = 0.f;
+    for (int j = 0; j < RL_CHUNKS; ++j) { a += part[2 * (b * RL_CHUNKS + j)]; c += part[2 * (b * RL_CHUNKS + j) + 1]; }
+    loss[1 + 2 * b] = a;
+    loss[2 + 2 * b] = c;
 }
 __global__ void k_rel_l2_grad(const float *__restrict__ pred, const float *__restrict__ tgt, float *__restrict__ dpred,
                               float *__restrict__ loss, int nb, long nper) {
@@ -291,7 +334,10 @@ extern "C" int mpc_gemm_bf16(const mpc_gemm *g, void *stream) {
     hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, *g);
     if (hipGetLastError() != hipSuccess) return MPC_EHIP;
     if (g->ksplit > 1) {
-        hipLaunchKernelGGL(k_gemm_reduce, dim3(grid_for((long)g->M * g->N * nbat)), dim3(256), 0, s, *g);
+        const long outs = (long)g->M * g->N * nbat;
+        if ((outs + RD_OUT - 1) / RD_OUT > 0x7fffffffL) return MPC_EINVAL;
+        hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((outs + RD_OUT - 1) / RD_OUT)), dim3(RD_OUT * RD_SUB), 0, s,
+                           *g);
         if (hipGetLastError() != hipSuccess) return MPC_EHIP;
     }
     return 0;
@@ -321,8 +367,14 @@ extern "C" int mpc_fno_act_bwd(const float *z, const float *dout, float *dz, lon
 
 extern "C" int mpc_fno_bias_grad(const float *d, float *db, long nb, int nc, long np, int accumulate, void *stream) {
     if (!d || !db || nb <= 0 || nc <= 0 || np <= 0) return MPC_EINVAL;
-    hipLaunchKernelGGL(k_bias_grad, dim3(nc), dim3(256), 0, (hipStream_t)stream, d, db, nb, nc, np, accumulate);
-    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+    hipStream_t s = (hipStream_t)stream;
+    float *part = nullptr;
+    if (hipMallocAsync((void **)&part, sizeof(float) * nc * BG_CHUNKS, s) != hipSuccess) return MPC_ENOMEM;
+    hipLaunchKernelGGL(k_bias_part, dim3(nc, BG_CHUNKS), dim3(256), 0, s, d, part, nb, nc, np);
+    hipLaunchKernelGGL(k_bias_sum, dim3((nc + 63) / 64), dim3(64), 0, s, part, db, nc, accumulate);
+    const bool ok = hipGetLastError() == hipSuccess;
+    (void)hipFreeAsync(part, s);
+    return ok ? 0 : MPC_EHIP;
 }
 
 extern "C" int mpc_fno_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2,
@@ -336,12 +388,15 @@ extern "C" int mpc_fno_rel_l2(const float *pred, const float *tgt, float *dpred,
                               void *stream) {
     if (!pred || !tgt || !dpred || !loss || nb <= 0 || nper <= 0) return MPC_EINVAL;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_rel_l2_sums, dim3(nb), dim3(256), 0, s, pred, tgt, loss, nper);
-    if (hipGetLastError() != hipSuccess) return MPC_EHIP;
+    float *part = nullptr;
+    if (hipMallocAsync((void **)&part, sizeof(float) * 2 * nb * RL_CHUNKS, s) != hipSuccess) return MPC_ENOMEM;
+    hipLaunchKernelGGL(k_rel_l2_part, dim3(nb, RL_CHUNKS), dim3(256), 0, s, pred, tgt, part, nper);
+    hipLaunchKernelGGL(k_rel_l2_sums, dim3((nb + 63) / 64), dim3(64), 0, s, part, loss, nb);
     hipLaunchKernelGGL(k_rel_l2_grad, dim3(grid_for(nper * nb)), dim3(256), 0, s, pred, tgt, dpred, loss, nb, nper);
-    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+    const bool ok = hipGetLastError() == hipSuccess;
+    (void)hipFreeAsync(part, s);
+    return ok ? 0 : MPC_EHIP;
 }
-
 /* device buffers for the surrogate's host driver (the default stream) */
 extern "C" int mpc_dev_alloc(long bytes, void **ptr) {
     if (!ptr || bytes <= 0) return MPC_EINVAL;

@@ -84,6 +84,7 @@ template <class T> int launch_to_reference(const cx<T> *src, double *dst, int nx
                                            hipStream_t s);
 int launch_maxabs(const double *v, long n, double *part, int nb, hipStream_t s);
 template <class T> int launch_scale(cx<T> *f, long n, T s, hipStream_t st);
+template <class T> int launch_planar(cx<T> *f, float *planes, long n, double s, int to_planes, hipStream_t st);
 template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines,
                                          double *line_power, double *temp_part, double *spec_part,
                                          int nblocks, hipStream_t s);

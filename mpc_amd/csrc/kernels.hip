@@ -95,6 +95,26 @@ template <class T> int launch_scale(cx<T> *f, long n, T s, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template int launch_scale<double>(cx<double> *, long, double, hipStream_t);
+
+/* resident field <-> two float planes [re | im] of n points (same point order) */
+template <class T>
+__global__ void k_planar(cx<T> *__restrict__ f, float *__restrict__ planes, long n, double s, int to_planes) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (to_planes) {
+            const cx<T> v = f[i];
+            planes[i] = (float)((double)v.re * s);
+            planes[n + i] = (float)((double)v.im * s);
+        } else {
+            f[i] = mk((T)((double)planes[i] * s), (T)((double)planes[n + i] * s));
+        }
+    }
+}
+template <class T> int launch_planar(cx<T> *f, float *planes, long n, double s, int to_planes, hipStream_t st) {
+    hipLaunchKernelGGL(k_planar<T>, dim3(1024), dim3(256), 0, st, f, planes, n, s, to_planes);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_planar<double>(cx<double> *, float *, long, double, int, hipStream_t);
+template int launch_planar<float>(cx<float> *, float *, long, double, int, hipStream_t);
 template int launch_scale<float>(cx<float> *, long, float, hipStream_t);
 
 /* ---------------------------------------------------------------------- */
@@ -165,6 +185,7 @@ static int diag_go(const cx<T> *f, const cx<T> *tw, long nlines, double *lp, dou
 template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines, double *lp,
                                          double *tp, double *sp, int nb, hipStream_t s) {
     switch (nt) {
+    case 256: return diag_go<T, 256>(f, tw, nlines, lp, tp, sp, nb, s);
     case 512: return diag_go<T, 512>(f, tw, nlines, lp, tp, sp, nb, s);
     case 1024: return diag_go<T, 1024>(f, tw, nlines, lp, tp, sp, nb, s);
     case 2048: return diag_go<T, 2048>(f, tw, nlines, lp, tp, sp, nb, s);
@@ -260,6 +281,7 @@ template <class T, int NT> static int ioniz_go(const cx<T> *f, long nlines, cons
 }
 template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const PlasmaK &pk, cx<double> *out, hipStream_t s) {
     switch (nt) {
+    case 256: return ioniz_go<T, 256>(f, nlines, pk, out, s);
     case 512: return ioniz_go<T, 512>(f, nlines, pk, out, s);
     case 1024: return ioniz_go<T, 1024>(f, nlines, pk, out, s);
     case 2048: return ioniz_go<T, 2048>(f, nlines, pk, out, s);

@@ -199,7 +199,7 @@ k_tline_batch(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart, i
     tline_work<T, G, PLASMA, false>(a, g, a.nlines, g);
 }
 
-bool tline_supported(int nt) { return nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
+bool tline_supported(int nt) { return nt == 256 || nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
 int tline_variants(void) { return 2; }
 
 template <class T, class G, bool PL, bool PERSIST>
@@ -241,6 +241,7 @@ static int tline_batch_go(const TArgs<T> *d, const long *gstart, int n, long nbl
 template <class T>
 int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long nblocks, int nt, bool plasma, hipStream_t s) {
     switch (nt) {
+    case 256:  return tline_batch_go<T, 256>(d, gstart, n, nblocks, plasma, s);
     case 512:  return tline_batch_go<T, 512>(d, gstart, n, nblocks, plasma, s);
     case 1024: return tline_batch_go<T, 1024>(d, gstart, n, nblocks, plasma, s);
     case 2048: return tline_batch_go<T, 2048>(d, gstart, n, nblocks, plasma, s);
@@ -254,6 +255,7 @@ template int launch_tline_batch<float>(const TArgs<float> *, const long *, int, 
 template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks) {
     if (variant < 0 || variant >= tline_variants()) return MPC_EINVAL;
     switch (nt) {
+    case 256:  return tline_v<T, 256>(a, plasma, variant, s, nblocks);
     case 512:  return tline_v<T, 512>(a, plasma, variant, s, nblocks);
     case 1024: return tline_v<T, 1024>(a, plasma, variant, s, nblocks);
     case 2048: return tline_v<T, 2048>(a, plasma, variant, s, nblocks);
@@ -268,6 +270,7 @@ template int launch_tline<float>(const TArgs<float> &, int, bool, int, hipStream
 int tline_lines_per_block(int nt, bool plasma, int prec) {
     (void)plasma; (void)prec;
     switch (nt) {
+    case 256: return TGeo<256>::LPB;
     case 512: return TGeo<512>::LPB;
     case 1024: return TGeo<1024>::LPB;
     case 2048: return TGeo<2048>::LPB;

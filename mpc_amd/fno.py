@@ -128,8 +128,8 @@ class Ops:
         # long reductions with few output tiles: split K over workgroups
         tiles = ((M + 63) // 64) * ((N + 63) // 64) * b1[0] * b2[0]
         ks = 1
-        if tiles < 256 and K >= 4096:
-            ks = min(256, max(1, K // 2048), max(1, self.ws.numel() // (M * N * b1[0] * b2[0])))
+        if tiles < 512 and K >= 4096:
+            ks = min(max(1, 2048 // tiles), max(1, K // 1024), max(1, self.ws.numel() // (M * N * b1[0] * b2[0])))
         g.ksplit = ks
         g.W = _ptr(self.ws) if ks > 1 else None
         check(lib().mpc_gemm_bf16(C.cast(C.pointer(g), C.c_void_p), _stream()), "mpc_gemm_bf16")
@@ -381,28 +381,30 @@ class FnoGpu:
         B = x.shape[0]
         npts = c.X * c.Y * c.T
         W, H = c.width, c.hidden
+        # activations live in named buffers that persist between calls (no
+        # allocation per call); with keep they are per layer, for the backward
         st = {"x": x}
         shp = (B, W, c.X, c.Y, c.T)
-        v = DArr(shp)
-        z0 = self._b("z0", shp)
+        v = self._b("v0", shp)
+        z0 = self._b("zP", shp)
         self._pw(self.p["P"], x, z0, W, c.cin, npts, B)
         self.ops.bias_act(z0, self.p["bP"], v, B, W, npts, False)
         for l in range(c.layers):
-            z = DArr(shp) if keep else self._b("z", shp)
-            V3 = DArr((B, W, c.mx, c.my, 2 * c.mt)) if keep else None
+            z = self._b(f"z{l}" if keep else "z", shp)
+            V3 = self._b(f"V3{l}", (B, W, c.mx, c.my, 2 * c.mt)) if keep else None
             self._spectral_fwd(v, l, z, V3)
             self._pw(self.p[f"W{l}"], v, z, W, W, npts, B, beta=1.0)
-            vn = DArr(shp)
+            vn = self._b(f"v{l + 1}" if keep else f"v{(l + 1) % 2}", shp)
             act = l < c.layers - 1
             self.ops.bias_act(z, self.p[f"b{l}"], vn, B, W, npts, act)
             if keep:
                 st[f"v{l}"], st[f"z{l}"], st[f"V3{l}"] = v, z, V3
             v = vn
-        hz = DArr((B, H, c.X, c.Y, c.T))
-        h = DArr((B, H, c.X, c.Y, c.T))
+        hz = self._b("hz", (B, H, c.X, c.Y, c.T))
+        h = self._b("h", (B, H, c.X, c.Y, c.T))
         self._pw(self.p["Q1"], v, hz, H, W, npts, B)
         self.ops.bias_act(hz, self.p["q1"], h, B, H, npts, True)
-        y = DArr((B, c.cout, c.X, c.Y, c.T))
+        y = self._b("y", (B, c.cout, c.X, c.Y, c.T))
         for b in range(B):                                       # residual: pred = x[:, :2] + ...
             y.copy_from(x, c.cout * npts, dst_off=b * c.cout * npts, src_off=b * c.cin * npts)
         self._pw(self.p["Q2"], h, y, c.cout, H, npts, B, beta=1.0)
@@ -423,14 +425,13 @@ class FnoGpu:
         # head
         self._pw_bwd_weight(dpred, st["h"], self.g["Q2"], c.cout, H, npts, B)
         o.bias_grad(dpred, self.g["q2"], B, c.cout, npts)
-        dh = DArr(st["h"].shape)
+        dh = self._b("dh", st["h"].shape)
         self._pw_bwd_input(self.p["Q2"], dpred, dh, c.cout, H, npts, B)
         o.act_bwd(st["hz"], dh, dh, True)
         self._pw_bwd_weight(dh, st["vL"], self.g["Q1"], H, W, npts, B)
         o.bias_grad(dh, self.g["q1"], B, H, npts)
-        dv = DArr(st["vL"].shape)
+        dv = self._b("dv", st["vL"].shape)
         self._pw_bwd_input(self.p["Q1"], dh, dv, H, W, npts, B)
-        del dh
         # Fourier layers, last to first
         for l in reversed(range(c.layers)):
             act = l < c.layers - 1
@@ -449,7 +450,7 @@ class FnoGpu:
         """relative L2 loss mean_b |pred-tgt|^2/|tgt|^2; gradients into self.g"""
         pred = self.forward(x, keep=True)
         dpred = self._b("dpred", pred.shape)
-        loss = zeros((1 + 2 * x.shape[0],))
+        loss = self._b("loss", (1 + 2 * x.shape[0],))
         self.ops.rel_l2(pred, tgt, dpred, loss)
         self.backward(dpred)
         return float(loss.numpy()[0])

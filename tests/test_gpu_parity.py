@@ -23,7 +23,8 @@ def _case(nx, nt, **kw):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
-@pytest.mark.parametrize("nx,nt,plasm,sst", [(64, 512, 1, 1), (64, 512, 0, 0), (128, 1024, 1, 1), (64, 2048, 1, 1)])
+@pytest.mark.parametrize("nx,nt,plasm,sst", [(64, 512, 1, 1), (64, 512, 0, 0), (128, 1024, 1, 1), (64, 2048, 1, 1),
+                                            (64, 256, 1, 1)])
 def test_run_matches_oracle(gpu_lib, prec, tol, nx, nt, plasm, sst):
     print(f"case run nx={nx} nt={nt} plasm={plasm} sst={sst} prec={prec}", flush=True)
     inp, A0 = _case(nx, nt, plasm=plasm, sst_on=sst, steps_per_pass=3, passes=3)
