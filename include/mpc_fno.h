@@ -50,6 +50,48 @@ int mpc_sizeof_gemm(void);   /* sizeof(mpc_gemm), for bindings to check their la
  */
 int mpc_fno_combine(const float *in, float *out, long nslice, int m, int nb, int h, int sign, int bwd, void *stream);
 
+/*
+ * Pointwise channel map with its epilogue (FP32):
+ *   z[b][o][p] = sum_i w[o*ldo + i*ldi] x[b*x_bs + i*np + p]
+ *                + beta * zin[b*zin_bs + o*np + p] + bias[o]
+ *   out[b][o][p] = GELU(z) if act else z   (out, zs: batch stride cout*np)
+ * zs (NULL: not stored) receives z; zin is read only if beta != 0; bias may
+ * be NULL; out may alias zin or zs; w and bias must not alias an output.
+ * cin in {1, 2, 4, 8, 16, 32, 64}, nb <= 65535.
+ */
+typedef struct {
+    int nb, cin, cout;
+    long np;
+    const float *x;
+    long x_bs;
+    const float *w;
+    long ldo, ldi;
+    const float *bias;
+    const float *zin;
+    long zin_bs;
+    float beta;
+    float *zs;
+    float *out;
+    int act;
+} mpc_pw;
+
+int mpc_fno_pointwise(const mpc_pw *a, void *stream);
+int mpc_sizeof_pw(void);    /* sizeof(mpc_pw), for bindings to check their layout */
+
+/*
+ * Per-mode complex channel mixing of the spectral convolution (FP32).
+ * V, U: [nb][w][nxy][2 mt] floats (re at kt, im at mt + kt of each kxy
+ * block); Rr, Ri: [w][w][nxy][mt] (input channel, output channel, mode).
+ * mpc_fno_mix:     U[b][o] = sum_i V[b][i] (Rr + i Ri)[i][o]   per mode.
+ * mpc_fno_mix_bwd: dV[b][i] = sum_o dU[b][o] conj(R[i][o]) (assigned) and
+ *                  dR[i][o] += sum_b conj(V[b][i]) dU[b][o] (accumulated).
+ * nb <= 8.
+ */
+int mpc_fno_mix(const float *V, const float *Rr, const float *Ri, float *U, int nb, int w, int nxy, int mt,
+                void *stream);
+int mpc_fno_mix_bwd(const float *V, const float *dU, const float *Rr, const float *Ri, float *dV, float *dRr,
+                    float *dRi, int nb, int w, int nxy, int mt, void *stream);
+
 /* z[b][c][p] += bias[c] (bias may be NULL), then out = GELU(z) if act else z */
 int mpc_fno_bias_act(float *z, const float *bias, float *out, long nb, int nc, long np, int act, void *stream);
 /* dz = dout * GELU'(z) if act else dout */

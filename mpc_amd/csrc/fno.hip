@@ -61,16 +61,17 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(mpc_gemm g) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) acc[p][q] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+    /* software pipeline: the next k tile's global loads are in flight while
+     * the matrix cores work on the current one from LDS */
+    float ra[8], rb[8];
+    auto load_tile = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             int ii, kk;
             if (a_krow) { kk = t % TK; ii = t / TK + 8 * r; }
             else        { ii = t % TM; kk = t / TM + 4 * r; }
             const int gi = i0 + ii, gk = k0 + kk;
-            float v = 0.f;
-            if (gi < g.M && gk < kend) v = A[(long)gi * g.rsA + (long)gk * g.csA];
-            As[ii][kk] = to_bf16(v);
+            ra[r] = (gi < g.M && gk < kend) ? A[(long)gi * g.rsA + (long)gk * g.csA] : 0.f;
         }
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -78,11 +79,27 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(mpc_gemm g) {
             if (b_ncol) { jj = t % TN; kk = t / TN + 4 * r; }
             else        { kk = t % TK; jj = t / TK + 8 * r; }
             const int gj = j0 + jj, gk = k0 + kk;
-            float v = 0.f;
-            if (gj < g.N && gk < kend) v = B[(long)gk * g.rsB + (long)gj * g.csB];
-            Bs[jj][kk] = to_bf16(v);
+            rb[r] = (gj < g.N && gk < kend) ? B[(long)gk * g.rsB + (long)gj * g.csB] : 0.f;
+        }
+    };
+    if (kbeg < kend) load_tile(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int ii, kk;
+            if (a_krow) { kk = t % TK; ii = t / TK + 8 * r; }
+            else        { ii = t % TM; kk = t / TM + 4 * r; }
+            As[ii][kk] = to_bf16(ra[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int jj, kk;
+            if (b_ncol) { jj = t % TN; kk = t / TN + 4 * r; }
+            else        { kk = t % TK; jj = t / TK + 8 * r; }
+            Bs[jj][kk] = to_bf16(rb[r]);
         }
         __syncthreads();
+        if (k0 + TK < kend) load_tile(k0 + TK);
 #pragma unroll
         for (int ks = 0; ks < TK; ks += 16) {
             const int kq = ks + 4 * (lane >> 4);
@@ -215,6 +232,181 @@ __global__ void k_act_bwd(const float *__restrict__ z, const float *__restrict__
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
         dz[x] = act ? dout[x] * gelu_d(z[x]) : dout[x];
 }
+/*
+ * Pointwise channel map (a 1x1 convolution) with its epilogue fused:
+ *   z[b][o][p] = sum_i w[o*ldo + i*ldi] x[b][i][p] + beta zin[b][o][p] + bias[o]
+ *   out[b][o][p] = GELU(z) (act) or z;  zs, if given, receives z.
+ * FP32 on the vector ALUs: at <= 64 channels a point costs cin*cout FMAs
+ * against 4 (cin + cout (+ cout)) bytes of HBM traffic, so the map is bound
+ * by HBM, and a matrix-core tiling would only add staging.  One thread per
+ * point (coalesced along p); the weights are wave-uniform (scalar loads).
+ * out may alias zin or zs (each element is read before it is written, by the
+ * same thread).
+ */
+/* a float in the constant address space: loads through such a pointer are
+ * known not to be clobbered by the kernel's stores, so uniform ones become
+ * scalar loads */
+typedef __attribute__((address_space(4))) const float cfloat;
+
+template <int CIN, int OC, bool UNIT>
+__global__ void __launch_bounds__(256) k_pointwise(mpc_pw a, const float *__restrict__ w,
+                                                   const float *__restrict__ bias) {
+    /* w and bias are read-only and wave-uniform: they load through the
+     * scalar unit into SGPRs, the FMAs' scalar operand (no LDS traffic) */
+    /* no early exit: the tail lanes of the last workgroup load a valid
+     * point and skip their stores, so the control flow stays uniform and the
+     * weight loads stay scalar */
+    const long p0 = blockIdx.x * 256L + threadIdx.x;
+    const bool live = p0 < a.np;
+    const long p = live ? p0 : a.np - 1;
+    const int b = blockIdx.y;
+    float xr[CIN];
+    const float *xb = a.x + b * a.x_bs + p;
+#pragma unroll
+    for (int i = 0; i < CIN; ++i) xr[i] = xb[(long)i * a.np];
+    const long ob = (long)b * a.cout * a.np + p;
+    const float *zb = a.beta != 0.f ? a.zin + b * a.zin_bs + p : nullptr;
+    /* outputs in chunks of OC (cout is a multiple of OC): the next chunk's
+     * residual loads are issued before this chunk's stores */
+    float zc[OC], zn[OC];
+#pragma unroll
+    for (int j = 0; j < OC; ++j) zc[j] = zb ? zb[(long)j * a.np] : 0.f;
+    for (int o0 = 0; o0 < a.cout; o0 += OC) {
+        float acc[OC];
+#pragma unroll
+        for (int j = 0; j < OC; ++j) {
+            const cfloat *wo = (const cfloat *)w + (o0 + j) * a.ldo;
+            float s = 0.f;
+#pragma unroll
+            for (int i = 0; i < CIN; ++i) s = fmaf(wo[UNIT ? i : i * a.ldi], xr[i], s);
+            acc[j] = s;
+        }
+        const bool more = zb && o0 + OC < a.cout;
+#pragma unroll
+        for (int j = 0; j < OC; ++j) zn[j] = more ? zb[(long)(o0 + OC + j) * a.np] : 0.f;
+#pragma unroll
+        for (int j = 0; j < OC; ++j) {
+            const int o = o0 + j;
+            float z = acc[j] + a.beta * zc[j];
+            if (bias) z += bias[o];
+            const float y = a.act ? gelu(z) : z;
+            if (live) {
+                if (a.zs) a.zs[ob + (long)o * a.np] = z;
+                a.out[ob + (long)o * a.np] = y;
+            }
+            zc[j] = zn[j];
+        }
+    }
+}
+
+template <int CIN, bool UNIT> static void launch_pointwise_u(const mpc_pw *a, dim3 grid, hipStream_t s) {
+    if (a->cout % 8 == 0) hipLaunchKernelGGL((k_pointwise<CIN, 8, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
+    else if (a->cout % 2 == 0)
+        hipLaunchKernelGGL((k_pointwise<CIN, 2, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
+    else hipLaunchKernelGGL((k_pointwise<CIN, 1, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
+}
+/* unit input stride (w[o][i] rows): the weight loads merge into wide scalar loads */
+template <int CIN> static void launch_pointwise(const mpc_pw *a, dim3 grid, hipStream_t s) {
+    if (a->ldi == 1) launch_pointwise_u<CIN, true>(a, grid, s);
+    else launch_pointwise_u<CIN, false>(a, grid, s);
+}
+
+/*
+ * Per-mode complex channel mixing of the spectral convolution, FP32.
+ * V, U: [b][channel][kxy][2 mt] (re at kt, im at mt + kt); R = Rr + i Ri:
+ * [i][o][kxy][mt].  Forward U[b][o] = sum_i V[b][i] R[i][o] (per mode).
+ * One thread per (mode, o); the weights stream once (coalesced along the
+ * mode), V is re-read per o from cache.
+ */
+constexpr int MIX_BMAX = 8;
+__global__ void __launch_bounds__(256) k_mix(const float *__restrict__ V, const float *__restrict__ Rr,
+                                             const float *__restrict__ Ri, float *__restrict__ U, int nb, int w,
+                                             int nxy, int mt) {
+    const int nm = nxy * mt;
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    const int o = blockIdx.y;
+    if (m >= nm) return;
+    const int kxy = m / mt, kt = m % mt;
+    const long vm = (long)kxy * 2 * mt + kt;          /* re offset of the mode in a channel's block */
+    const long cs = (long)nxy * 2 * mt;               /* channel stride in V, U */
+    float ur[MIX_BMAX], ui[MIX_BMAX];
+#pragma unroll
+    for (int b = 0; b < MIX_BMAX; ++b) ur[b] = ui[b] = 0.f;
+    for (int i = 0; i < w; ++i) {
+        const long ri = ((long)i * w + o) * nm + m;
+        const float rr = Rr[ri], rim = Ri[ri];
+#pragma unroll
+        for (int b = 0; b < MIX_BMAX; ++b) {
+            if (b < nb) {
+                const float *v = V + ((long)b * w + i) * cs + vm;
+                const float vr = v[0], vi = v[mt];
+                ur[b] = fmaf(vr, rr, fmaf(-vi, rim, ur[b]));
+                ui[b] = fmaf(vr, rim, fmaf(vi, rr, ui[b]));
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < MIX_BMAX; ++b) {
+        if (b < nb) {
+            float *u = U + ((long)b * w + o) * cs + vm;
+            u[0] = ur[b];
+            u[mt] = ui[b];
+        }
+    }
+}
+/*
+ * Its backward, one thread per (mode, i):  dV[b][i] = sum_o dU[b][o] conj(R[i][o])
+ * (assigned) and dR[i][o] += sum_b conj(V[b][i]) dU[b][o] (accumulated).
+ */
+__global__ void __launch_bounds__(256) k_mix_bwd(const float *__restrict__ V, const float *__restrict__ dU,
+                                                 const float *__restrict__ Rr, const float *__restrict__ Ri,
+                                                 float *__restrict__ dV, float *__restrict__ dRr,
+                                                 float *__restrict__ dRi, int nb, int w, int nxy, int mt) {
+    const int nm = nxy * mt;
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y;
+    if (m >= nm) return;
+    const int kxy = m / mt, kt = m % mt;
+    const long vm = (long)kxy * 2 * mt + kt;
+    const long cs = (long)nxy * 2 * mt;
+    float vr[MIX_BMAX], vi[MIX_BMAX], gr[MIX_BMAX], gi[MIX_BMAX];
+#pragma unroll
+    for (int b = 0; b < MIX_BMAX; ++b) {
+        vr[b] = vi[b] = gr[b] = gi[b] = 0.f;
+        if (b < nb) {
+            const float *v = V + ((long)b * w + i) * cs + vm;
+            vr[b] = v[0];
+            vi[b] = v[mt];
+        }
+    }
+    for (int o = 0; o < w; ++o) {
+        const long ri = ((long)i * w + o) * nm + m;
+        const float rr = Rr[ri], rim = Ri[ri];
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int b = 0; b < MIX_BMAX; ++b) {
+            if (b < nb) {
+                const float *d = dU + ((long)b * w + o) * cs + vm;
+                const float dr = d[0], di = d[mt];
+                gr[b] = fmaf(dr, rr, fmaf(di, rim, gr[b]));        /* dU conj(R) */
+                gi[b] = fmaf(di, rr, fmaf(-dr, rim, gi[b]));
+                sr = fmaf(vr[b], dr, fmaf(vi[b], di, sr));         /* conj(V) dU */
+                si = fmaf(vr[b], di, fmaf(-vi[b], dr, si));
+            }
+        }
+        dRr[ri] += sr;
+        dRi[ri] += si;
+    }
+#pragma unroll
+    for (int b = 0; b < MIX_BMAX; ++b) {
+        if (b < nb) {
+            float *d = dV + ((long)b * w + i) * cs + vm;
+            d[0] = gr[b];
+            d[mt] = gi[b];
+        }
+    }
+}
+
 /* db[c] (+)= sum over b, p of d[b][c][p], in two stages: workgroup (c, j)
  * sums a fixed chunk of channel c into part[c][j]; then one thread per
  * channel adds its chunks in order (deterministic) */
@@ -341,6 +533,47 @@ extern "C" int mpc_gemm_bf16(const mpc_gemm *g, void *stream) {
         if (hipGetLastError() != hipSuccess) return MPC_EHIP;
     }
     return 0;
+}
+
+extern "C" int mpc_sizeof_pw(void) { return (int)sizeof(mpc_pw); }
+
+extern "C" int mpc_fno_pointwise(const mpc_pw *a, void *stream) {
+    if (!a || !a->x || !a->w || !a->out || a->nb <= 0 || a->nb > 65535 || a->cin <= 0 || a->cout <= 0 ||
+        a->np <= 0 || (a->beta != 0.f && !a->zin))
+        return MPC_EINVAL;
+    const dim3 grid((unsigned)((a->np + 255) / 256), (unsigned)a->nb);
+    hipStream_t s = (hipStream_t)stream;
+    switch (a->cin) {
+    case 1: launch_pointwise<1>(a, grid, s); break;
+    case 2: launch_pointwise<2>(a, grid, s); break;
+    case 4: launch_pointwise<4>(a, grid, s); break;
+    case 8: launch_pointwise<8>(a, grid, s); break;
+    case 16: launch_pointwise<16>(a, grid, s); break;
+    case 32: launch_pointwise<32>(a, grid, s); break;
+    case 64: launch_pointwise<64>(a, grid, s); break;
+    default: return MPC_EINVAL;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+extern "C" int mpc_fno_mix(const float *V, const float *Rr, const float *Ri, float *U, int nb, int w, int nxy,
+                           int mt, void *stream) {
+    if (!V || !Rr || !Ri || !U || nb <= 0 || nb > MIX_BMAX || w <= 0 || w > 65535 || nxy <= 0 || mt <= 0)
+        return MPC_EINVAL;
+    const dim3 grid((unsigned)((nxy * mt + 255) / 256), (unsigned)w);
+    hipLaunchKernelGGL(k_mix, grid, dim3(256), 0, (hipStream_t)stream, V, Rr, Ri, U, nb, w, nxy, mt);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+extern "C" int mpc_fno_mix_bwd(const float *V, const float *dU, const float *Rr, const float *Ri, float *dV,
+                               float *dRr, float *dRi, int nb, int w, int nxy, int mt, void *stream) {
+    if (!V || !dU || !Rr || !Ri || !dV || !dRr || !dRi || nb <= 0 || nb > MIX_BMAX || w <= 0 || w > 65535 ||
+        nxy <= 0 || mt <= 0)
+        return MPC_EINVAL;
+    const dim3 grid((unsigned)((nxy * mt + 255) / 256), (unsigned)w);
+    hipLaunchKernelGGL(k_mix_bwd, grid, dim3(256), 0, (hipStream_t)stream, V, dU, Rr, Ri, dV, dRr, dRi, nb, w, nxy,
+                       mt);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 
 extern "C" int mpc_fno_combine(const float *in, float *out, long nslice, int m, int nb, int h, int sign, int bwd,

@@ -40,6 +40,16 @@ class MpcGemm(C.Structure):
                 ("alpha", C.c_float), ("beta", C.c_float)]
 
 
+class MpcPw(C.Structure):
+    _fields_ = [("nb", C.c_int), ("cin", C.c_int), ("cout", C.c_int), ("np", C.c_long),
+                ("x", C.c_void_p), ("x_bs", C.c_long), ("w", C.c_void_p), ("ldo", C.c_long), ("ldi", C.c_long),
+                ("bias", C.c_void_p), ("zin", C.c_void_p), ("zin_bs", C.c_long), ("beta", C.c_float),
+                ("zs", C.c_void_p), ("out", C.c_void_p), ("act", C.c_int)]
+
+
+MIX_BMAX = 8          # samples per mode-mixing launch
+
+
 class DArr:
     """A float32 device array owned by this object (mpc_dev_alloc)."""
 
@@ -111,6 +121,8 @@ class Ops:
             raise RuntimeError("the FNO surrogate runs on the GPU only (no device visible)")
         if lib().mpc_sizeof_gemm() != C.sizeof(MpcGemm):
             raise RuntimeError("mpc_gemm layout mismatch between library and binding")
+        if lib().mpc_sizeof_pw() != C.sizeof(MpcPw):
+            raise RuntimeError("mpc_pw layout mismatch between library and binding")
         self.ws = DArr((1 << 22,))                # split-K workspace (16 MiB)
 
     def gemm(self, M, N, K, A, rsA, csA, B, rsB, csB, Cm, rsC, csC, alpha=1.0, beta=0.0,
@@ -137,6 +149,40 @@ class Ops:
     def combine(self, src, dst, nslice, m, nb, h, sign, bwd=False):
         check(lib().mpc_fno_combine(_ptr(src), _ptr(dst), nslice, m, nb, h, sign, 1 if bwd else 0, _stream()),
               "mpc_fno_combine")
+
+    def pointwise(self, x, cin, w, ldo, ldi, out, cout, nb, npts, bias=None, zin=None, beta=0.0, zs=None,
+                  act=False, x_bs=None, zin_bs=None):
+        """out[b][o] = act(sum_i w[o*ldo + i*ldi] x[b][i] + beta zin[b][o] + bias[o]) per point; zs gets the
+        pre-activation"""
+        a = MpcPw()
+        a.nb, a.cin, a.cout, a.np = nb, cin, cout, npts
+        a.x, a.x_bs = _ptr(x), cin * npts if x_bs is None else x_bs
+        a.w, a.ldo, a.ldi = _ptr(w), ldo, ldi
+        a.bias = _ptr(bias) if bias is not None else None
+        a.zin = _ptr(zin) if zin is not None else None
+        a.zin_bs = cout * npts if zin_bs is None else zin_bs
+        a.beta = beta
+        a.zs = _ptr(zs) if zs is not None else None
+        a.out = _ptr(out)
+        a.act = 1 if act else 0
+        check(lib().mpc_fno_pointwise(C.cast(C.pointer(a), C.c_void_p), _stream()), "mpc_fno_pointwise")
+
+    def mix(self, V, Rr, Ri, U, nb, w, nxy, mt):
+        """U = V R per mode (complex), in launches of at most MIX_BMAX samples"""
+        per = w * nxy * 2 * mt
+        for b0 in range(0, nb, MIX_BMAX):
+            n = min(MIX_BMAX, nb - b0)
+            check(lib().mpc_fno_mix(_ptr(V, b0 * per), _ptr(Rr), _ptr(Ri), _ptr(U, b0 * per), n, w, nxy, mt,
+                                    _stream()), "mpc_fno_mix")
+
+    def mix_bwd(self, V, dU, Rr, Ri, dV, dRr, dRi, nb, w, nxy, mt):
+        """dV = dU conj(R) per mode; dR += conj(V) dU summed over samples"""
+        per = w * nxy * 2 * mt
+        for b0 in range(0, nb, MIX_BMAX):
+            n = min(MIX_BMAX, nb - b0)
+            check(lib().mpc_fno_mix_bwd(_ptr(V, b0 * per), _ptr(dU, b0 * per), _ptr(Rr), _ptr(Ri),
+                                        _ptr(dV, b0 * per), _ptr(dRr), _ptr(dRi), n, w, nxy, mt, _stream()),
+                  "mpc_fno_mix_bwd")
 
     def bias_act(self, z, bias, out, nb, nc, npts, act):
         check(lib().mpc_fno_bias_act(_ptr(z), _ptr(bias) if bias is not None else None, _ptr(out), nb, nc, npts,
@@ -290,19 +336,10 @@ class FnoGpu:
         # inverse t
         o.gemm(B * W * X * Y, T, n2, V6, n2, 1, M["Gt"], T, 1, out, T, 1)
 
-    def _mix(self, V, Rr, Ri, U, B, beta=0.0):
+    def _mix(self, V, Rr, Ri, U, B):
         """U = V . R per mode (complex), V/U: (B, W, mx, my, 2mt), R: (W, W, mx*my, mt)"""
-        c, o = self.cfg, self.ops
-        W, mx, my, mt = c.width, c.mx, c.my, c.mt
-        nm2, nm = mx * my * 2 * mt, mx * my * mt
-        b1 = (mx * my, 2 * mt, mt, 2 * mt)
-        b2 = (mt, 1, 1, 1)
-        a = dict(b1=b1, b2=b2)
-        # Ur = Vr Rr - Vi Ri ; Ui = Vr Ri + Vi Rr
-        o.gemm(B, W, W, V, W * nm2, nm2, Rr, W * nm, nm, U, W * nm2, nm2, beta=beta, **a)
-        o.gemm(B, W, W, V, W * nm2, nm2, Ri, W * nm, nm, U, W * nm2, nm2, alpha=-1.0, beta=1.0, offA=mt, **a)
-        o.gemm(B, W, W, V, W * nm2, nm2, Ri, W * nm, nm, U, W * nm2, nm2, beta=beta, offC=mt, **a)
-        o.gemm(B, W, W, V, W * nm2, nm2, Rr, W * nm, nm, U, W * nm2, nm2, beta=1.0, offA=mt, offC=mt, **a)
+        c = self.cfg
+        self.ops.mix(V, Rr, Ri, U, B, c.width, c.mx * c.my, c.mt)
 
     def _spectral_bwd(self, dz, l, V3, dv):
         """dv += S_l^T(dz); accumulates dRr_l, dRi_l from the saved mode coefficients V3"""
@@ -341,32 +378,14 @@ class FnoGpu:
         o.gemm(B * W * X * Y, T, n2, dC1, n2, 1, M["Ft"], 1, n2, dv, T, 1, beta=1.0)
 
     def _mix_bwd(self, V, dU, Rr, Ri, dV, dRr, dRi, B):
-        c, o = self.cfg, self.ops
-        W, mx, my, mt = c.width, c.mx, c.my, c.mt
-        nm2, nm = mx * my * 2 * mt, mx * my * mt
-        a = dict(b1=(mx * my, 2 * mt, mt, 2 * mt), b2=(mt, 1, 1, 1))
-        # dVr = dUr Rr^T + dUi Ri^T ; dVi = -dUr Ri^T + dUi Rr^T
-        o.gemm(B, W, W, dU, W * nm2, nm2, Rr, nm, W * nm, dV, W * nm2, nm2, **a)
-        o.gemm(B, W, W, dU, W * nm2, nm2, Ri, nm, W * nm, dV, W * nm2, nm2, beta=1.0, offA=mt, **a)
-        o.gemm(B, W, W, dU, W * nm2, nm2, Ri, nm, W * nm, dV, W * nm2, nm2, alpha=-1.0, offC=mt, **a)
-        o.gemm(B, W, W, dU, W * nm2, nm2, Rr, nm, W * nm, dV, W * nm2, nm2, beta=1.0, offA=mt, offC=mt, **a)
-        # dRr = Vr^T dUr + Vi^T dUi ; dRi = -Vi^T dUr + Vr^T dUi   (W x B . B x W per mode)
-        g = dict(b1=(mx * my, 2 * mt, 2 * mt, mt), b2=(mt, 1, 1, 1))
-        o.gemm(W, W, B, V, nm2, W * nm2, dU, W * nm2, nm2, dRr, W * nm, nm, beta=1.0, **g)
-        o.gemm(W, W, B, V, nm2, W * nm2, dU, W * nm2, nm2, dRr, W * nm, nm, beta=1.0, offA=mt, offB=mt, **g)
-        o.gemm(W, W, B, V, nm2, W * nm2, dU, W * nm2, nm2, dRi, W * nm, nm, alpha=-1.0, beta=1.0, offA=mt, **g)
-        o.gemm(W, W, B, V, nm2, W * nm2, dU, W * nm2, nm2, dRi, W * nm, nm, beta=1.0, offB=mt, **g)
+        """dV = dU . R^H per mode; dR += V^H . dU"""
+        c = self.cfg
+        self.ops.mix_bwd(V, dU, Rr, Ri, dV, dRr, dRi, B, c.width, c.mx * c.my, c.mt)
 
     # ---- pointwise maps --------------------------------------------------------
-    def _pw(self, Wm, x, out, nout, nin, npts, B, beta=0.0):
-        """out[b] (nout x npts) = Wm (nout x nin) . x[b] (+ beta out)"""
-        self.ops.gemm(nout, npts, nin, Wm, nin, 1, x, npts, 1, out, npts, 1, beta=beta,
-                      b1=(B, 0, nin * npts, nout * npts))
-
-    def _pw_bwd_input(self, Wm, dout, dx, nout, nin, npts, B, beta=0.0):
-        """dx[b] = Wm^T . dout[b]"""
-        self.ops.gemm(nin, npts, nout, Wm, 1, nin, dout, npts, 1, dx, npts, 1, beta=beta,
-                      b1=(B, 0, nout * npts, nin * npts))
+    def _pw_bwd_input(self, Wm, dout, dx, nout, nin, npts, B):
+        """dx[b] = Wm^T . dout[b]  (Wm: nout x nin, row-major)"""
+        self.ops.pointwise(dout, nout, Wm, 1, nin, dx, nin, B, npts)
 
     def _pw_bwd_weight(self, dout, x, dW, nout, nin, npts, B):
         """dW += sum_b dout[b] . x[b]^T"""
@@ -386,29 +405,26 @@ class FnoGpu:
         st = {"x": x}
         shp = (B, W, c.X, c.Y, c.T)
         v = self._b("v0", shp)
-        z0 = self._b("zP", shp)
-        self._pw(self.p["P"], x, z0, W, c.cin, npts, B)
-        self.ops.bias_act(z0, self.p["bP"], v, B, W, npts, False)
+        o = self.ops
+        o.pointwise(x, c.cin, self.p["P"], c.cin, 1, v, W, B, npts, bias=self.p["bP"])
         for l in range(c.layers):
             z = self._b(f"z{l}" if keep else "z", shp)
             V3 = self._b(f"V3{l}", (B, W, c.mx, c.my, 2 * c.mt)) if keep else None
             self._spectral_fwd(v, l, z, V3)
-            self._pw(self.p[f"W{l}"], v, z, W, W, npts, B, beta=1.0)
             vn = self._b(f"v{l + 1}" if keep else f"v{(l + 1) % 2}", shp)
             act = l < c.layers - 1
-            self.ops.bias_act(z, self.p[f"b{l}"], vn, B, W, npts, act)
+            o.pointwise(v, W, self.p[f"W{l}"], W, 1, vn, W, B, npts, bias=self.p[f"b{l}"], zin=z, beta=1.0,
+                        zs=z if keep else None, act=act)
             if keep:
                 st[f"v{l}"], st[f"z{l}"], st[f"V3{l}"] = v, z, V3
             v = vn
-        hz = self._b("hz", (B, H, c.X, c.Y, c.T))
+        hz = self._b("hz", (B, H, c.X, c.Y, c.T)) if keep else None
         h = self._b("h", (B, H, c.X, c.Y, c.T))
-        self._pw(self.p["Q1"], v, hz, H, W, npts, B)
-        self.ops.bias_act(hz, self.p["q1"], h, B, H, npts, True)
+        o.pointwise(v, W, self.p["Q1"], W, 1, h, H, B, npts, bias=self.p["q1"], zs=hz if keep else None, act=True)
         y = self._b("y", (B, c.cout, c.X, c.Y, c.T))
-        for b in range(B):                                       # residual: pred = x[:, :2] + ...
-            y.copy_from(x, c.cout * npts, dst_off=b * c.cout * npts, src_off=b * c.cin * npts)
-        self._pw(self.p["Q2"], h, y, c.cout, H, npts, B, beta=1.0)
-        self.ops.bias_act(y, self.p["q2"], y, B, c.cout, npts, False)
+        # pred = x[:, :cout] + Q2 h + q2
+        o.pointwise(h, H, self.p["Q2"], H, 1, y, c.cout, B, npts, bias=self.p["q2"], zin=x, beta=1.0,
+                    zin_bs=c.cin * npts)
         if keep:
             st["vL"], st["hz"], st["h"] = v, hz, h
             self._st = st

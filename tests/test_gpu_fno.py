@@ -4,6 +4,9 @@
   transposed operands, two-level batches, alpha/beta and split K (FP32
   accumulation: 1e-4 of the result's norm).
 * k_combine forward against numpy and its bwd map as the exact transpose.
+* the fused pointwise channel map (bias, residual, GELU, pre-activation
+  store, transposed weights) and the per-mode complex mixing with its
+  backward against numpy (FP32: 1e-5 of the largest term).
 * the network's forward against the FP64 CPU network (fno_cpu) to BF16
   tolerance (3e-2), and the hand-written backward's gradient of every
   parameter tensor against central finite differences of the CPU loss along
@@ -76,6 +79,73 @@ def test_combine_and_its_transpose(gpu_lib):
         lhs = float(np.sum(V.numpy().astype(np.float64) * dV))
         rhs = float(np.sum(P.astype(np.float64) * dP.numpy()))
         assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def _gelu(z):
+    from scipy.special import erf
+    return 0.5 * z * (1 + erf(z / np.sqrt(2)))
+
+
+@pytest.mark.parametrize("cin,cout,act", [(4, 32, False), (32, 32, True), (64, 2, False), (2, 64, True)])
+def test_pointwise_channel_map(gpu_lib, cin, cout, act):
+    """fused 1x1 map against numpy: plain, transposed weights, residual with
+    its own batch stride, pre-activation store, in-place output"""
+    from mpc_amd.fno import DArr, Ops
+    print(f"case pointwise cin={cin} cout={cout} act={act}", flush=True)
+    rng = np.random.default_rng(cin * 100 + cout)
+    nb, npts = 3, 1000
+    x = rng.standard_normal((nb, cin, npts)).astype(np.float32)
+    Wm = rng.standard_normal((cout, cin)).astype(np.float32) / np.sqrt(cin)
+    bias = rng.standard_normal(cout).astype(np.float32)
+    zin = rng.standard_normal((nb, cout + 1, npts)).astype(np.float32)   # batch stride (cout+1)*npts
+    ops = Ops()
+    f = (lambda z: _gelu(z)) if act else (lambda z: z)
+    z_ref = np.einsum("oi,bip->bop", Wm.astype(np.float64), x) + 0.5 * zin[:, :cout] + bias[None, :, None]
+    out, zs = DArr((nb, cout, npts)), DArr((nb, cout, npts))
+    ops.pointwise(_t(x), cin, _t(Wm), cin, 1, out, cout, nb, npts, bias=_t(bias), zin=_t(zin), beta=0.5, zs=zs,
+                  act=act, zin_bs=(cout + 1) * npts)
+    assert np.abs(zs.numpy() - z_ref).max() <= 1e-5 * np.abs(z_ref).max()
+    assert np.abs(out.numpy() - f(z_ref)).max() <= 1e-5 * np.abs(z_ref).max()
+    # transposed weights (the backward's W^T: weight of input o for output i
+    # at o*cin + i), no bias, output in place of zin
+    y = rng.standard_normal((nb, cout, npts)).astype(np.float32)
+    acc = rng.standard_normal((nb, cin, npts)).astype(np.float32)
+    d = _t(acc)
+    ops.pointwise(_t(y), cout, _t(Wm), 1, cin, d, cin, nb, npts, zin=d, beta=1.0)
+    ref = np.einsum("oi,bop->bip", Wm.astype(np.float64), y) + acc
+    assert np.abs(d.numpy() - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_mode_mixing_and_its_adjoint(gpu_lib):
+    """per-mode complex mixing against numpy; backward as the exact adjoint
+    (<U(V), dU> = <V, dV>) and the weight gradient against numpy, with more
+    samples than one launch takes"""
+    from mpc_amd.fno import DArr, Ops
+    rng = np.random.default_rng(3)
+    nb, w, nxy, mt = 10, 6, 5, 7
+    V = rng.standard_normal((nb, w, nxy, 2 * mt)).astype(np.float32)
+    dU = rng.standard_normal((nb, w, nxy, 2 * mt)).astype(np.float32)
+    Rr = rng.standard_normal((w, w, nxy, mt)).astype(np.float32)
+    Ri = rng.standard_normal((w, w, nxy, mt)).astype(np.float32)
+    ops = Ops()
+    U = DArr((nb, w, nxy, 2 * mt))
+    ops.mix(_t(V), _t(Rr), _t(Ri), U, nb, w, nxy, mt)
+    Vc = V[..., :mt].astype(np.float64) + 1j * V[..., mt:]
+    R = Rr.astype(np.float64) + 1j * Ri
+    Uc = np.einsum("bikm,iokm->bokm", Vc, R)
+    got = U.numpy()
+    assert np.abs(got[..., :mt] - Uc.real).max() <= 1e-5 * np.abs(Uc).max()
+    assert np.abs(got[..., mt:] - Uc.imag).max() <= 1e-5 * np.abs(Uc).max()
+    g0 = rng.standard_normal(Rr.shape).astype(np.float32)
+    dV, dRr, dRi = DArr(V.shape), _t(g0), _t(g0)
+    ops.mix_bwd(_t(V), _t(dU), _t(Rr), _t(Ri), dV, dRr, dRi, nb, w, nxy, mt)
+    lhs = float(np.sum(got.astype(np.float64) * dU))
+    rhs = float(np.sum(V.astype(np.float64) * dV.numpy()))
+    assert abs(lhs - rhs) <= 1e-5 * np.abs(got * dU).sum()
+    dUc = dU[..., :mt].astype(np.float64) + 1j * dU[..., mt:]
+    dR = np.einsum("bikm,bokm->iokm", np.conj(Vc), dUc)
+    assert np.abs(dRr.numpy() - g0 - dR.real).max() <= 1e-4 * np.abs(dR).max()
+    assert np.abs(dRi.numpy() - g0 - dR.imag).max() <= 1e-4 * np.abs(dR).max()
 
 
 def _small():
