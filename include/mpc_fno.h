@@ -78,6 +78,9 @@ typedef struct {
 int mpc_fno_pointwise(const mpc_pw *a, void *stream);
 int mpc_sizeof_pw(void);    /* sizeof(mpc_pw), for bindings to check their layout */
 
+/* out (cols x rows) = in (rows x cols)^T; in != out */
+int mpc_fno_transpose(const float *in, float *out, int rows, int cols, void *stream);
+
 /*
  * Per-mode complex channel mixing of the spectral convolution (FP32).
  * V, U: [nb][w][nxy][2 mt] floats (re at kt, im at mt + kt of each kxy

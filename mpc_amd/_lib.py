@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
         "mpc_fno_act_bwd": (C.c_int, [vp, vp, vp, C.c_long, C.c_int, vp]),
         "mpc_sizeof_pw": (C.c_int, []),
         "mpc_fno_pointwise": (C.c_int, [vp, vp]),
+        "mpc_fno_transpose": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
         "mpc_fno_mix": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "mpc_fno_mix_bwd": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "mpc_fno_bias_grad": (C.c_int, [vp, vp, C.c_long, C.c_int, C.c_long, C.c_int, vp]),

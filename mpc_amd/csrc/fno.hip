@@ -502,6 +502,15 @@ __global__ void k_rel_l2_grad(const float *__restrict__ pred, const float *__res
     }
 }
 
+/* out[j][i] = in[i][j] for a rows x cols matrix (small: weight matrices) */
+__global__ void k_transpose(const float *__restrict__ in, float *__restrict__ out, int rows, int cols) {
+    const long n = (long)rows * cols;
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(x / cols), j = (int)(x % cols);
+        out[(long)j * rows + i] = in[x];
+    }
+}
+
 __global__ void k_fill(float *__restrict__ p, long n, float v) {
     for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) p[x] = v;
 }
@@ -553,6 +562,13 @@ extern "C" int mpc_fno_pointwise(const mpc_pw *a, void *stream) {
     case 64: launch_pointwise<64>(a, grid, s); break;
     default: return MPC_EINVAL;
     }
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
+extern "C" int mpc_fno_transpose(const float *in, float *out, int rows, int cols, void *stream) {
+    if (!in || !out || in == out || rows <= 0 || cols <= 0) return MPC_EINVAL;
+    hipLaunchKernelGGL(k_transpose, dim3(grid_for((long)rows * cols)), dim3(256), 0, (hipStream_t)stream, in, out,
+                       rows, cols);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 

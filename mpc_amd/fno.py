@@ -167,6 +167,9 @@ class Ops:
         a.act = 1 if act else 0
         check(lib().mpc_fno_pointwise(C.cast(C.pointer(a), C.c_void_p), _stream()), "mpc_fno_pointwise")
 
+    def transpose(self, src, dst, rows, cols):
+        check(lib().mpc_fno_transpose(_ptr(src), _ptr(dst), rows, cols, _stream()), "mpc_fno_transpose")
+
     def mix(self, V, Rr, Ri, U, nb, w, nxy, mt):
         """U = V R per mode (complex), in launches of at most MIX_BMAX samples"""
         per = w * nxy * 2 * mt
@@ -384,8 +387,11 @@ class FnoGpu:
 
     # ---- pointwise maps --------------------------------------------------------
     def _pw_bwd_input(self, Wm, dout, dx, nout, nin, npts, B):
-        """dx[b] = Wm^T . dout[b]  (Wm: nout x nin, row-major)"""
-        self.ops.pointwise(dout, nout, Wm, 1, nin, dx, nin, B, npts)
+        """dx[b] = Wm^T . dout[b]  (Wm: nout x nin, row-major; its transpose is
+        formed first so the map reads unit-stride weight rows)"""
+        WT = self._b(f"WT{nout}x{nin}", (nin, nout))
+        self.ops.transpose(Wm, WT, nout, nin)
+        self.ops.pointwise(dout, nout, WT, nout, 1, dx, nin, B, npts)
 
     def _pw_bwd_weight(self, dout, x, dW, nout, nin, npts, B):
         """dW += sum_b dout[b] . x[b]^T"""
@@ -476,3 +482,49 @@ class FnoGpu:
         lr_t = lr * math.sqrt(1 - b2 ** self.step) / (1 - b1 ** self.step)
         for k in self.p:
             self.ops.adam(self.p[k], self.g[k], self.m[k], self.v[k], lr_t, b1, b2, eps, wd)
+
+
+def time_network(batch: int = 1, reps: int = 20, train_steps: int = 0, seed: int = 1) -> dict:
+    """host-clock timing of the network at its full size (synthetic input):
+    forward passes at `batch`, then `train_steps` training steps (forward,
+    backward, Adam) at the same batch"""
+    import time
+    cfg = FnoConfig()
+    net = FnoGpu(cfg, init_params(cfg, seed))
+    n = cfg.X * cfg.Y * cfg.T
+    rng = np.random.default_rng(seed)
+    x = DArr.of(rng.standard_normal((batch, cfg.cin, n)).astype(np.float32) * 0.1)
+    net.forward(x)
+    sync()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        net.forward(x)
+        sync()
+        ts.append(time.perf_counter() - t0)
+    out = {"batch": batch, "forward_ms": 1e3 * float(np.median(ts))}
+    if train_steps:
+        tgt = DArr.of(rng.standard_normal((batch, cfg.cout, n)).astype(np.float32) * 0.1)
+        net.loss_and_grad(x, tgt)
+        net.adam_step()
+        sync()
+        ts = []
+        for _ in range(train_steps):
+            t0 = time.perf_counter()
+            net.loss_and_grad(x, tgt)
+            net.adam_step()
+            sync()
+            ts.append(time.perf_counter() - t0)
+        out["train_step_ms"] = 1e3 * float(np.median(ts))
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="time the surrogate network at its full size")
+    ap.add_argument("--batch", type=int, default=1)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--train-steps", type=int, default=0)
+    a = ap.parse_args()
+    print(json.dumps(time_network(a.batch, a.reps, a.train_steps)), flush=True)

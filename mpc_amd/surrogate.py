@@ -205,11 +205,12 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
     sc = scalars_of(wl)
     errs, kinds = [], []
     t_assist = 0.0
+    d_cur = None                                 # diagnostics of the current state, when known
     for k in range(passes):
         t0 = time.perf_counter()
         kind = "plain" if k % plain_every == plain_every - 1 else "net"
         if kind == "net":
-            d_in = p.diagnostics()
+            d_in = d_cur if d_cur is not None else p.diagnostics()
             export(p, start, 0, scale)
             linear_pass(p, wl)
             export(p, x, 0, scale)
@@ -224,11 +225,13 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
             d_out = p.diagnostics()
             e_ok = abs(d_out["energy_sum"] / d_in["energy_sum"] - 1.0) <= energy_tol
             w_ok = abs(d_out["spec_rms_width"] / d_in["spec_rms_width"] - 1.0) <= width_tol
+            d_cur = d_out
             if not (e_ok and w_ok):
                 import_(p, start, 0, scale)      # back to the pass's input, then the plain pass
                 kind = "fallback"
         if kind != "net":
             p.run(k * S, S)
+            d_cur = None
         t_assist += time.perf_counter() - t0
         kinds.append(kind)
         export(p, cur, 0, scale)

@@ -114,6 +114,9 @@ def test_pointwise_channel_map(gpu_lib, cin, cout, act):
     ops.pointwise(_t(y), cout, _t(Wm), 1, cin, d, cin, nb, npts, zin=d, beta=1.0)
     ref = np.einsum("oi,bop->bip", Wm.astype(np.float64), y) + acc
     assert np.abs(d.numpy() - ref).max() <= 1e-5 * np.abs(ref).max()
+    WT = DArr((cin, cout))
+    ops.transpose(_t(Wm), WT, cout, cin)
+    assert np.array_equal(WT.numpy(), Wm.astype(np.float32).T)
 
 
 def test_mode_mixing_and_its_adjoint(gpu_lib):
