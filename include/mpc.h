@@ -133,12 +133,13 @@ int  mpc_variant_counts(int *tline_variants, int *axis_variants);
  * npasses), advance together.  Every sweep of a step is one launch per size
  * class covering all pulses of that class; with use_graph the launches of
  * each pass are captured once into a HIP graph and replayed for every pass
- * with the same launch list.  The x/y sweeps use each class's first
- * pulse's plan and tile order (variant % 16) in the one-workgroup-per-tile
- * form, the t sweep the one-workgroup-per-line-group form, so each pulse's
- * result is bit-identical to mpc_run on its context with t-line variant 1
- * and those x/y variants.  The batch does not own the contexts; destroy it
- * before them.
+ * with the same launch list.  Each launch takes its class's first pulse's
+ * kernel variants (mpc_set_variants_xy / mpc_autotune): the persistent
+ * variants walk the line groups or tiles of all pulses of the class with the
+ * workgroups that fit on the device at once, the others take one workgroup
+ * per group or tile.  Each pulse's result is bit-identical to mpc_run on its
+ * context with those variants.  The batch does not own the contexts; destroy
+ * it before them.
  */
 typedef struct mpc_batch_s mpc_batch;
 int  mpc_batch_create(mpc_ctx *const *ctxs, int n, mpc_batch **out);

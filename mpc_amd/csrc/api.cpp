@@ -515,7 +515,8 @@ struct BLaunch {
     int plasma;
     int variant;
     int n;              /* fields in the launch */
-    long nblocks;
+    long nblocks;       /* line groups ('t') or tiles ('x', 'y') over all fields */
+    long grid;          /* workgroups launched */
     const void *desc;
     const long *starts;
     int step;
@@ -567,11 +568,13 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
                 starts.push_back(tot);
                 tot += c->nxy / tline_lines_per_block(c->nt, true, b->prec);
             }
+            starts.push_back(tot);                           /* starts[n] = total */
+            L.variant = b->ctx[k.m[0]]->tvar;                 /* persistent (0) or per group (1) */
             L.desc = pool_put(b, d.data(), d.size() * sizeof(TArgs<T>));
         } else {
             const int axis = kind == 'x' ? 0 : 1;
             const mpc_ctx *c0 = b->ctx[k.m[0]];
-            L.variant = (axis == 0 ? c0->avar : c0->avar_y) % 16;     /* one workgroup per tile */
+            L.variant = axis == 0 ? c0->avar : c0->avar_y;            /* plan, tile order and persistence */
             const int G = axis_tile_g(b->prec, L.variant);
             std::vector<AArgs<T>> d;
             for (int p : k.m) {
@@ -589,10 +592,12 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
                 starts.push_back(tot);
                 tot += (long)nch * a.n_outer;
             }
+            starts.push_back(tot);                           /* starts[n] = total */
             L.desc = pool_put(b, d.data(), d.size() * sizeof(AArgs<T>));
         }
         L.starts = (const long *)pool_put(b, starts.data(), starts.size() * sizeof(long));
         L.nblocks = tot;
+        L.grid = kind == 't' ? tline_batch_grid<T>(tot, k.len, k.plasma != 0, L.variant) : tot;
         if (!L.desc || !L.starts) return MPC_ENOMEM;
         out.push_back(std::move(L));
     }
@@ -602,7 +607,7 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
 template <class T> static int batch_launch(mpc_batch *b, const BLaunch &L) {
     if (L.kind == 't')
         return launch_tline_batch<T>((const TArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.plasma != 0,
-                                     b->stream);
+                                     L.variant, b->stream);
     return launch_axis_batch<T>((const AArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.variant, b->stream);
 }
 
@@ -612,8 +617,8 @@ static std::string launch_key(const std::vector<BLaunch> &ls, size_t i0, size_t 
     for (size_t i = i0; i < i1; i++) {
         const BLaunch &L = ls[i];
         char buf[128];
-        snprintf(buf, sizeof buf, "%c%d.%d.%d.%d.%ld.%p.%p;", L.kind, L.len, L.plasma, L.variant, L.n, L.nblocks,
-                 L.desc, (const void *)L.starts);
+        snprintf(buf, sizeof buf, "%c%d.%d.%d.%d.%ld.%ld.%p.%p;", L.kind, L.len, L.plasma, L.variant, L.n, L.nblocks,
+                 L.grid, L.desc, (const void *)L.starts);
         k += buf;
     }
     return k;
@@ -662,10 +667,16 @@ template <class T> static int batch_run(mpc_batch *b, int step0, int nsteps, boo
         i0 = i1;
     }
     HIPCHK(hipStreamSynchronize(b->stream));
-    for (size_t p = 0; p < np; p++) {
-        mpc_ctx *c = b->ctx[p];
-        c->domain_freq = fend[p] ? 1 : 0;
-        if (c->p.plasm == 1) c->rho_nvalid = (int)(c->nxy / tline_lines_per_block(c->nt, true, b->prec));
+    for (size_t p = 0; p < np; p++) b->ctx[p]->domain_freq = fend[p] ? 1 : 0;
+    /* plasma maxima: a per-group launch wrote one slot per line group, a
+     * persistent one the first min(grid, groups) slots of each field */
+    for (const BLaunch &L : ls) {
+        if (L.kind != 't' || !L.plasma) continue;
+        for (int p : L.members) {
+            mpc_ctx *c = b->ctx[p];
+            const long groups = c->nxy / tline_lines_per_block(c->nt, true, b->prec);
+            c->rho_nvalid = (int)(L.variant == 0 ? std::min(L.grid, groups) : groups);
+        }
     }
     return 0;
 }

@@ -10,7 +10,14 @@
  * Built with machine-level loop-invariant code motion off, like tline.hip:
  * the persistent form's tile loop would otherwise keep every product of the
  * cached twiddle powers in registers.
+ *
+ * Multiply-adds are contracted per source expression only (the pragma below
+ * precedes the headers, so it covers the FFT code they define): the
+ * per-field and the batched kernels then round identically, whatever the
+ * code around the shared arithmetic.  These sweeps are bound by HBM, not by
+ * their FMAs.
  */
+#pragma clang fp contract(on)
 #include "device.h"
 
 namespace mpc {
@@ -40,6 +47,28 @@ template <class T> __device__ __forceinline__ void axis_tile_of(const AArgs<T> &
     const long rest = lin / a.tg;
     outer = rest % a.n_outer;
     t0 = ((rest / a.n_outer) * a.tg + lin % a.tg) * G;
+}
+
+/* the arithmetic of one tile (this thread's E elements of one line in v):
+ * FFT, x D(k), inverse FFT [x mirror] [x per-t factor p].  Shared by the
+ * per-field and the batched sweeps. */
+template <class T, int N, class PL, int G_, bool PERSIST, class Fft>
+__device__ __forceinline__ void axis_apply(const AArgs<T> &a, cx<T> (&v)[ACfg<T, N, PL, G_>::E], cx<T> p, long outer,
+                                           int tid, const cx<T> (&Dc)[ACfg<T, N, PL, G_>::E], Fft &fft) {
+    using C = ACfg<T, N, PL, G_>;
+    constexpr int E = C::E, TPL = C::TPL;
+    fft(v, std::integral_constant<int, -1>{});
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], PERSIST ? Dc[m] : a.D[tid + TPL * m]);
+    fft(v, std::integral_constant<int, +1>{});
+    if (a.M) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
+    }
+    if (a.Pt) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], p);
+    }
 }
 
 /*
@@ -107,18 +136,7 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
                 if (a.Pt) pn = a.Pt[tn + gcol];
             }
         }
-        fft(v, std::integral_constant<int, -1>{});
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], PERSIST ? Dc[m] : a.D[tid + TPL * m]);
-        fft(v, std::integral_constant<int, +1>{});
-        if (a.M) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
-        }
-        if (a.Pt) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], p);
-        }
+        axis_apply<T, N, PL, G_, PERSIST>(a, v, p, outer, tid, Dc, fft);
 #pragma unroll
         for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
         if (!PERSIST || lnext >= ntiles) break;
@@ -148,6 +166,98 @@ k_axis_batch(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, in
     const AArgs<T> a = d[lo];
     const long t = b - tstart[lo];
     axis_work<T, N, PL, G_, false>(a, t, (long)(a.nt / G_) * a.n_outer);
+}
+
+/*
+ * Persistent batched form: the grid is what fits on the device at once and
+ * workgroup w walks the concatenated tiles of every field in d[0..n) (field
+ * p owns [tstart[p], tstart[p + 1]), tstart[n] = total), reloading the
+ * field's arguments, twiddle powers and D(k) entries at each field boundary
+ * and prefetching the next tile across boundaries.  Same arithmetic per tile
+ * as the persistent per-field kernel (axis_apply), with each field's tile
+ * order.
+ */
+template <class T, int N, class PL, int G_>
+__global__ void __launch_bounds__((ACfg<T, N, PL, G_>::THREADS))
+k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, int n) {
+    using C = ACfg<T, N, PL, G_>;
+    using CO = ChainOf<T, N, C::E, typename PL::R>;
+    constexpr int E = C::E, TPL = C::TPL, G = C::G;
+    __shared__ __align__(16) cx<T> lds[N * G];
+    const int gcol0 = threadIdx.x % G, tid0 = threadIdx.x / G;
+    const long total = tstart[n];
+    long l = blockIdx.x;
+    if (l >= total) return;                           /* whole workgroup */
+    int p = 0;
+    {
+        int hi = n - 1;
+        while (p < hi) {
+            const int mid = (p + hi + 1) / 2;
+            if (tstart[mid] <= l) p = mid;
+            else hi = mid - 1;
+        }
+    }
+    AArgs<T> a = d[p];
+    typename CO::type twc;
+    cx<T> Dc[E];
+    auto load_field = [&] {
+        twc.load(a.tw, tid0);
+#pragma unroll
+        for (int m = 0; m < E; ++m) Dc[m] = a.D[tid0 + TPL * m];
+    };
+    load_field();
+    int tid = tid0, gcol = gcol0;
+    auto fft = [&](cx<T>(&x)[E], auto dir) {
+        constexpr int DIR = decltype(dir)::value;
+        CO::template fft<DIR, 0>(x, tid, lds + gcol, G, twc);
+    };
+
+    cx<T> nxt[E], pn = mk((T)1, (T)0);
+    {
+        long outer, t0;
+        axis_tile_of(a, l - tstart[p], outer, t0, G);
+        const cx<T> *b0 = a.field + outer * a.s_outer + t0 + gcol0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
+        if (a.Pt) pn = a.Pt[t0 + gcol0];
+    }
+    for (;;) {
+        tid = tid0;
+        gcol = gcol0;
+        asm volatile("" : "+v"(tid), "+v"(gcol));
+        opaque_chain(twc);
+        long outer, t0;
+        axis_tile_of(a, l - tstart[p], outer, t0, G);
+        cx<T> *base = a.field + outer * a.s_outer + t0 + gcol;
+        cx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        const cx<T> pf = pn;
+        const long ln = l + gridDim.x;
+        int q = p;
+        while (q + 1 < n && tstart[q + 1] <= ln) ++q;
+        if (ln < total) {
+            const AArgs<T> &an = d[q];
+            long on, tn;
+            axis_tile_of(an, ln - tstart[q], on, tn, G);
+            const cx<T> *bn = an.field + on * an.s_outer + tn + gcol;
+#pragma unroll
+            for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * an.s_line];
+            if (an.Pt) pn = an.Pt[tn + gcol];
+        }
+        axis_apply<T, N, PL, G_, true>(a, v, pf, outer, tid, Dc, fft);
+#pragma unroll
+        for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
+        if (ln >= total) break;
+        if (q != p) {
+            /* the next field's tables; the LDS of the last tile is read by
+             * now (the inverse FFT's last exchange precedes its last stage) */
+            p = q;
+            a = d[p];
+            load_field();
+        }
+        l = ln;
+    }
 }
 
 bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
@@ -203,25 +313,34 @@ int axis_tile_g(int precision, int variant) {
 int axis_tile_tg(int variant, int nchunks) { return axis_tg((variant / 4) % 4, nchunks); }
 
 template <class T, int N, class PL, int G>
-static int axis_batch_go(const AArgs<T> *d, const long *tstart, int n, long nblocks, hipStream_t s) {
+static int axis_batch_go(const AArgs<T> *d, const long *tstart, int n, long total, bool persist, hipStream_t s) {
     using C = ACfg<T, N, PL, G>;
-    if (n <= 0 || nblocks <= 0 || nblocks > 0x7fffffffL) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_axis_batch<T, N, PL, G>), dim3((unsigned)nblocks), dim3(C::THREADS), 0, s, d, tstart, n);
+    if (n <= 0 || total <= 0 || total > 0x7fffffffL) return MPC_EINVAL;
+    if (persist) {
+        static long resident = -1;              /* per instantiation */
+        if (resident < 0) resident = resident_blocks((const void *)k_axis_batch_p<T, N, PL, G>, C::THREADS, 0);
+        const long nb = (resident > 0 && resident < total) ? resident : total;
+        hipLaunchKernelGGL((k_axis_batch_p<T, N, PL, G>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, d, tstart, n);
+    } else {
+        hipLaunchKernelGGL((k_axis_batch<T, N, PL, G>), dim3((unsigned)total), dim3(C::THREADS), 0, s, d, tstart, n);
+    }
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T, int N>
-static int axis_batch_v(const AArgs<T> *d, const long *tstart, int n, long nblocks, int variant, hipStream_t s) {
+static int axis_batch_v(const AArgs<T> *d, const long *tstart, int n, long total, int variant, hipStream_t s) {
     constexpr int G = ATile<T>::G;
+    const bool persist = variant / 16 != 0;
     switch (variant % 4) {
-    case 1: return axis_batch_go<T, N, Plan<N>, G / 2>(d, tstart, n, nblocks, s);
-    case 2: return axis_batch_go<T, N, APlan8<N>, G / 2>(d, tstart, n, nblocks, s);
-    case 3: return axis_batch_go<T, N, APlan8<N>, G>(d, tstart, n, nblocks, s);
-    default: return axis_batch_go<T, N, Plan<N>, G>(d, tstart, n, nblocks, s);
+    case 1: return axis_batch_go<T, N, Plan<N>, G / 2>(d, tstart, n, total, persist, s);
+    case 2: return axis_batch_go<T, N, APlan8<N>, G / 2>(d, tstart, n, total, persist, s);
+    case 3: return axis_batch_go<T, N, APlan8<N>, G>(d, tstart, n, total, persist, s);
+    default: return axis_batch_go<T, N, Plan<N>, G>(d, tstart, n, total, persist, s);
     }
 }
 /* batched x/y sweep: d[p] carries each field's strides, tables, n_outer and
  * tg (as the variant's tile order gives them); the plan and tile width come
- * from variant % 4 */
+ * from variant % 4, persistence from variant / 16 (tstart holds n + 1
+ * entries, the last the total tile count) */
 template <class T>
 int launch_axis_batch(const AArgs<T> *d, const long *tstart, int n, long nblocks, int len, int variant, hipStream_t s) {
     if (variant < 0 || variant >= axis_variants()) return MPC_EINVAL;

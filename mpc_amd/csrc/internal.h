@@ -107,7 +107,10 @@ int tline_lines_per_block(int nt, bool plasma, int prec);
  * line group / tile of every field; descriptors and start offsets in device
  * memory) */
 template <class T>
-int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long nblocks, int nt, bool plasma, hipStream_t s);
+int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long total, int nt, bool plasma, int variant,
+                       hipStream_t s);
+/* workgroups a batched t-line launch over `total` line groups runs (variant 0: persistent) */
+template <class T> long tline_batch_grid(long total, int nt, bool plasma, int variant);
 template <class T>
 int launch_axis_batch(const AArgs<T> *d, const long *tstart, int n, long nblocks, int len, int variant, hipStream_t s);
 

@@ -26,6 +26,95 @@ namespace mpc {
  * behind the previous line's FFTs.  Without PERSIST the grid has one
  * workgroup per line group and every table is read per line.
  */
+/*
+ * The arithmetic of one line (this thread's E elements in v, the line's LDS
+ * at lraw): [t-FFT] [x P] -> nonlinear step -> t-FFT [x Q] [-> t-IFFT].
+ * prefetch() runs once the input has been consumed, so a persistent caller
+ * can start loading its next line there.  Shared by the per-field and the
+ * batched sweeps, so both do the same arithmetic per line.
+ */
+template <class T, class G, bool PLASMA, bool PERSIST, class Fft, class Pre>
+__device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], int tid, unsigned char *lraw,
+                                            const T (&wk)[G::E], double &rmax, Fft &fft, Pre &&prefetch) {
+    constexpr int E = G::E, TPL = G::TPL;
+    using Fwd = std::integral_constant<int, -1>;
+    using Inv = std::integral_constant<int, +1>;
+    if (!a.in_freq) fft(v, Fwd{});
+    if (a.P) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
+    }
+    prefetch();
+
+    if (a.do_nl) {
+        cx<T> d[E];
+        if (a.sst_on) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const T w = PERSIST ? wk[m] : a.wfft[tid + TPL * m];
+                d[m] = mk(w * v[m].im, -w * v[m].re);   /* -i omega A(omega) */
+            }
+            fft(d, Inv{});                             /* dA */
+        }
+        fft(v, Inv{});                                 /* A  */
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
+            if (a.sst_on) {
+                /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232) */
+                const cx<T> t1 = cmulc(d[m], v[m]);
+                const cx<T> t2 = cmulc(v[m], d[m]);
+                const T sre = a.sst_coef * ((T)2 * t1.re + t2.re);
+                const T sim = a.sst_coef * ((T)2 * t1.im + t2.im);
+                d[m] = mk(-sre, a.gamma * a2 - sim);
+            } else {
+                d[m] = mk((T)0, a.gamma * a2);
+            }
+        }
+        if constexpr (PLASMA) {
+            const double half_rho = 0.5 * a.pk.rho_nt;
+            plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
+                               [&](int m, double F, double tau) {
+                                   /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
+                                   const double z = half_rho * F;
+                                   d[m].re -= (T)z;
+                                   d[m].im -= (T)(a.pk.w0 * tau * z);
+                               });
+        }
+        /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T er = exp_t<T>(a.dz * d[m].re);
+            T sn, cs;
+            sincos_t<T>(a.dz * d[m].im, &sn, &cs);
+            const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
+            v[m] = mk(er * z.re, er * z.im);
+        }
+        fft(v, Fwd{});
+    }
+    if (a.Q) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
+    }
+    if (a.out_time) fft(v, Inv{});
+}
+
+/* workgroup max of the ionisation fraction over its lines, into *slot (all
+ * threads call it; the LDS scratch is free again on return) */
+template <class G> __device__ __forceinline__ void tline_flush_rho(double rmax, double *slot) {
+    __shared__ double red[32];
+    double r = rmax;
+    for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_down(r, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int i = 0; i < (G::THREADS + 63) / 64; ++i) mx = fmax(mx, red[i]);
+        *slot = mx;
+    }
+    __syncthreads();
+}
+
 /* the work of one workgroup: line groups grp, grp + gstride, ... of the
  * field a describes (one group unless PERSIST); its plasma maximum goes to
  * a.rho_block[rho_slot] */
@@ -58,8 +147,6 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
         if constexpr (PERSIST) CO::template fft<DIR, G::PAD, C::BOFF>(x, tid, L, 1, twc);
         else line_fft<T, NT, PL, DIR, G::PAD, C::BOFF>(x, tid, L, 1, a.tw);
     };
-    using Fwd = std::integral_constant<int, -1>;
-    using Inv = std::integral_constant<int, +1>;
 
     cx<T> nxt[E];
     {
@@ -82,90 +169,23 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
         cx<T> v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
-        if (!a.in_freq) fft(v, Fwd{});
-        if (a.P) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
-        }
         const long gnext = grp + gstride;
-        if constexpr (PERSIST) {
-            if (gnext < ngroups) {
-                const cx<T> *gn = a.field + (gnext * LPB + lslot) * NT;
+        tline_line<T, G, PLASMA, PERSIST>(a, v, tid, lraw, wk, rmax, fft, [&] {
+            if constexpr (PERSIST) {
+                if (gnext < ngroups) {
+                    const cx<T> *gn = a.field + (gnext * LPB + lslot) * NT;
 #pragma unroll
-                for (int m = 0; m < E; ++m) nxt[m] = gn[tid + TPL * m];
-            }
-        }
-
-        if (a.do_nl) {
-            cx<T> d[E];
-            if (a.sst_on) {
-#pragma unroll
-                for (int m = 0; m < E; ++m) {
-                    const T w = PERSIST ? wk[m] : a.wfft[tid + TPL * m];
-                    d[m] = mk(w * v[m].im, -w * v[m].re);   /* -i omega A(omega) */
-                }
-                fft(d, Inv{});                             /* dA */
-            }
-            fft(v, Inv{});                                 /* A  */
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
-                if (a.sst_on) {
-                    /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232) */
-                    const cx<T> t1 = cmulc(d[m], v[m]);
-                    const cx<T> t2 = cmulc(v[m], d[m]);
-                    const T sre = a.sst_coef * ((T)2 * t1.re + t2.re);
-                    const T sim = a.sst_coef * ((T)2 * t1.im + t2.im);
-                    d[m] = mk(-sre, a.gamma * a2 - sim);
-                } else {
-                    d[m] = mk((T)0, a.gamma * a2);
+                    for (int m = 0; m < E; ++m) nxt[m] = gn[tid + TPL * m];
                 }
             }
-            if constexpr (PLASMA) {
-                const double half_rho = 0.5 * a.pk.rho_nt;
-                plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
-                                   [&](int m, double F, double tau) {
-                                       /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
-                                       const double z = half_rho * F;
-                                       d[m].re -= (T)z;
-                                       d[m].im -= (T)(a.pk.w0 * tau * z);
-                                   });
-            }
-            /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const T er = exp_t<T>(a.dz * d[m].re);
-                T sn, cs;
-                sincos_t<T>(a.dz * d[m].im, &sn, &cs);
-                const cx<T> z = mk(v[m].re * cs - v[m].im * sn, v[m].re * sn + v[m].im * cs);
-                v[m] = mk(er * z.re, er * z.im);
-            }
-            fft(v, Fwd{});
-        }
-        if (a.Q) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
-        }
-        if (a.out_time) fft(v, Inv{});
+        });
 #pragma unroll
         for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
         if (!PERSIST || gnext >= ngroups) break;
         grp = gnext;
     }
     if constexpr (PLASMA) {
-        if (a.rho_block && a.do_nl) {
-            /* workgroup max of the ionisation fraction over its lines */
-            __shared__ double red[32];
-            double r = rmax;
-            for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_down(r, o));
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double mx = 0.0;
-                for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mx = fmax(mx, red[i]);
-                a.rho_block[rho_slot] = mx;
-            }
-        }
+        if (a.rho_block && a.do_nl) tline_flush_rho<G>(rmax, a.rho_block + rho_slot);
     }
 }
 
@@ -199,6 +219,113 @@ k_tline_batch(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart, i
     tline_work<T, G, PLASMA, false>(a, g, a.nlines, g);
 }
 
+/*
+ * Persistent batched form: the grid is what fits on the device at once, and
+ * workgroup w walks the concatenated line groups of every field in d[0..n)
+ * (global group w, w + gridDim.x, ...; field p owns [gstart[p],
+ * gstart[p + 1]), gstart[n] = total).  At each field boundary it reloads the
+ * field's arguments, twiddle powers and Omega grid, and flushes its plasma
+ * maximum for the field it leaves to slot j = its first group in that field,
+ * so the slots [0, min(grid, groups)) of each field are written exactly
+ * once.  The next group is prefetched across field boundaries too.  The
+ * per-line arithmetic is the persistent per-field kernel's (tline_line).
+ */
+template <class T, class G, bool PLASMA>
+__device__ __forceinline__ void tline_work_batch(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart,
+                                                 int n) {
+    using C = TCfg<T, G, PLASMA>;
+    using PL = typename G::PL;
+    using CO = ChainOf<T, G::NT, C::E, typename PL::R>;
+    constexpr int NT = G::NT;
+    constexpr int E = C::E, TPL = C::TPL, LPB = C::LPB;
+    __shared__ __align__(16) unsigned char smem[LPB * C::LINE_BYTES];
+    const int lslot0 = threadIdx.x / TPL, tid0 = threadIdx.x % TPL;
+    const long total = gstart[n];
+    long g = blockIdx.x;
+    if (g >= total) return;                           /* whole workgroup */
+    int p = 0;
+    {
+        int hi = n - 1;
+        while (p < hi) {
+            const int mid = (p + hi + 1) / 2;
+            if (gstart[mid] <= g) p = mid;
+            else hi = mid - 1;
+        }
+    }
+    TArgs<T> a = d[p];
+    long first = g - gstart[p];
+    int tid = tid0, lslot = lslot0;
+
+    typename CO::type twc;
+    T wk[E];
+    auto load_field = [&] {
+        twc.load(a.tw, tid);
+        if (a.do_nl && a.sst_on) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) wk[m] = a.wfft[tid + TPL * m];
+        }
+    };
+    load_field();
+    unsigned char *lraw;
+    cx<T> *L;
+    auto fft = [&](cx<T>(&x)[E], auto dir) {
+        constexpr int DIR = decltype(dir)::value;
+        CO::template fft<DIR, G::PAD, C::BOFF>(x, tid, L, 1, twc);
+    };
+
+    cx<T> nxt[E];
+    {
+        const cx<T> *g0 = a.field + ((g - gstart[p]) * LPB + lslot) * NT;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = g0[tid + TPL * m];
+    }
+    double rmax = 0.0;
+    for (;;) {
+        tid = tid0;
+        lslot = lslot0;
+        asm volatile("" : "+v"(tid), "+v"(lslot));
+        opaque_chain(twc);
+        lraw = smem + lslot * C::LINE_BYTES;
+        L = reinterpret_cast<cx<T> *>(lraw);
+        cx<T> *gp = a.field + ((g - gstart[p]) * LPB + lslot) * NT;
+        cx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        const long gn = g + gridDim.x;
+        int pn = p;
+        while (pn + 1 < n && gstart[pn + 1] <= gn) ++pn;
+        tline_line<T, G, PLASMA, true>(a, v, tid, lraw, wk, rmax, fft, [&] {
+            if (gn < total) {
+                const cx<T> *fn = pn == p ? a.field : d[pn].field;
+                const cx<T> *src = fn + ((gn - gstart[pn]) * LPB + lslot) * NT;
+#pragma unroll
+                for (int m = 0; m < E; ++m) nxt[m] = src[tid + TPL * m];
+            }
+        });
+#pragma unroll
+        for (int m = 0; m < E; ++m) gp[tid + TPL * m] = v[m];
+        if (gn >= total || pn != p) {
+            /* leaving field p */
+            if constexpr (PLASMA) {
+                if (a.rho_block && a.do_nl) tline_flush_rho<G>(rmax, a.rho_block + first);
+            }
+            rmax = 0.0;
+            if (gn >= total) break;
+            p = pn;
+            a = d[p];
+            first = gn - gstart[p];
+            load_field();
+        }
+        g = gn;
+    }
+}
+
+template <class T, class G, bool PLASMA>
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), sizeof(T) == 4 ? 2 : 1)
+k_tline_batch_p(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart, int n) {
+    tline_work_batch<T, G, PLASMA>(d, gstart, n);
+}
+
 bool tline_supported(int nt) { return nt == 256 || nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
 int tline_variants(void) { return 2; }
 
@@ -226,31 +353,67 @@ template <class T, int NT> static int tline_v(const TArgs<T> &a, bool plasma, in
         return plasma ? tline_go<T, GEO, true, true>(a, s, nb) : tline_go<T, GEO, false, true>(a, s, nb);
     return plasma ? tline_go<T, GEO, true, false>(a, s, nb) : tline_go<T, GEO, false, false>(a, s, nb);
 }
-template <class T, int NT>
-static int tline_batch_go(const TArgs<T> *d, const long *gstart, int n, long nblocks, bool plasma, hipStream_t s) {
+/* grid of a batched launch over `total` line groups: one workgroup per group
+ * (variant 1) or what fits on the device at once (variant 0, persistent) */
+template <class T, int NT> static long tline_batch_grid_t(long total, bool plasma, int variant) {
     using GEO = TGeo<NT>;
-    if (n <= 0 || nblocks <= 0 || nblocks > 0x7fffffffL) return MPC_EINVAL;
-    if (plasma)
-        hipLaunchKernelGGL((k_tline_batch<T, GEO, true>), dim3((unsigned)nblocks), dim3(TCfg<T, GEO, true>::THREADS),
+    if (variant != 0) return total;
+    static long res[2] = {-1, -1};              /* per instantiation and plasma switch */
+    long &r = res[plasma ? 1 : 0];
+    if (r < 0)
+        r = plasma ? resident_blocks((const void *)k_tline_batch_p<T, GEO, true>, TCfg<T, GEO, true>::THREADS, 0)
+                   : resident_blocks((const void *)k_tline_batch_p<T, GEO, false>, TCfg<T, GEO, false>::THREADS, 0);
+    return (r > 0 && r < total) ? r : total;
+}
+template <class T, int NT>
+static int tline_batch_go(const TArgs<T> *d, const long *gstart, int n, long total, bool plasma, int variant,
+                          hipStream_t s) {
+    using GEO = TGeo<NT>;
+    if (n <= 0 || total <= 0 || total > 0x7fffffffL) return MPC_EINVAL;
+    const long nb = tline_batch_grid_t<T, NT>(total, plasma, variant);
+    if (variant == 0) {
+        if (plasma)
+            hipLaunchKernelGGL((k_tline_batch_p<T, GEO, true>), dim3((unsigned)nb), dim3(TCfg<T, GEO, true>::THREADS),
+                               0, s, d, gstart, n);
+        else
+            hipLaunchKernelGGL((k_tline_batch_p<T, GEO, false>), dim3((unsigned)nb),
+                               dim3(TCfg<T, GEO, false>::THREADS), 0, s, d, gstart, n);
+    } else if (plasma) {
+        hipLaunchKernelGGL((k_tline_batch<T, GEO, true>), dim3((unsigned)nb), dim3(TCfg<T, GEO, true>::THREADS),
                            0, s, d, gstart, n);
-    else
-        hipLaunchKernelGGL((k_tline_batch<T, GEO, false>), dim3((unsigned)nblocks), dim3(TCfg<T, GEO, false>::THREADS),
+    } else {
+        hipLaunchKernelGGL((k_tline_batch<T, GEO, false>), dim3((unsigned)nb), dim3(TCfg<T, GEO, false>::THREADS),
                            0, s, d, gstart, n);
+    }
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T>
-int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long nblocks, int nt, bool plasma, hipStream_t s) {
+int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long total, int nt, bool plasma, int variant,
+                       hipStream_t s) {
     switch (nt) {
-    case 256:  return tline_batch_go<T, 256>(d, gstart, n, nblocks, plasma, s);
-    case 512:  return tline_batch_go<T, 512>(d, gstart, n, nblocks, plasma, s);
-    case 1024: return tline_batch_go<T, 1024>(d, gstart, n, nblocks, plasma, s);
-    case 2048: return tline_batch_go<T, 2048>(d, gstart, n, nblocks, plasma, s);
-    case 4096: return tline_batch_go<T, 4096>(d, gstart, n, nblocks, plasma, s);
+    case 256:  return tline_batch_go<T, 256>(d, gstart, n, total, plasma, variant, s);
+    case 512:  return tline_batch_go<T, 512>(d, gstart, n, total, plasma, variant, s);
+    case 1024: return tline_batch_go<T, 1024>(d, gstart, n, total, plasma, variant, s);
+    case 2048: return tline_batch_go<T, 2048>(d, gstart, n, total, plasma, variant, s);
+    case 4096: return tline_batch_go<T, 4096>(d, gstart, n, total, plasma, variant, s);
     default: return MPC_EUNSUPP;
     }
 }
-template int launch_tline_batch<double>(const TArgs<double> *, const long *, int, long, int, bool, hipStream_t);
-template int launch_tline_batch<float>(const TArgs<float> *, const long *, int, long, int, bool, hipStream_t);
+template int launch_tline_batch<double>(const TArgs<double> *, const long *, int, long, int, bool, int, hipStream_t);
+template int launch_tline_batch<float>(const TArgs<float> *, const long *, int, long, int, bool, int, hipStream_t);
+
+template <class T> long tline_batch_grid(long total, int nt, bool plasma, int variant) {
+    switch (nt) {
+    case 256:  return tline_batch_grid_t<T, 256>(total, plasma, variant);
+    case 512:  return tline_batch_grid_t<T, 512>(total, plasma, variant);
+    case 1024: return tline_batch_grid_t<T, 1024>(total, plasma, variant);
+    case 2048: return tline_batch_grid_t<T, 2048>(total, plasma, variant);
+    case 4096: return tline_batch_grid_t<T, 4096>(total, plasma, variant);
+    default: return total;
+    }
+}
+template long tline_batch_grid<double>(long, int, bool, int);
+template long tline_batch_grid<float>(long, int, bool, int);
 
 template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int variant, hipStream_t s, int *nblocks) {
     if (variant < 0 || variant >= tline_variants()) return MPC_EINVAL;

@@ -2,11 +2,10 @@
 graph per pass) against per-pulse runs and the oracle.
 
 Bar: each pulse's batched result is bit-identical to mpc_run on its own
-context with the one-workgroup-per-tile kernels the batch runs (t-line
-variant 1, the same x/y plan); one pulse is also checked against the FP64
-oracle at 1e-10.  (The persistent variants do the same arithmetic per line
-but are compiled separately and need not round identically; the variant test
-checks them against the oracle.)
+context with the same kernel variants (the batch takes each size class's
+first pulse's: persistent variants walk the line groups or tiles of every
+pulse in the launch, the others take one workgroup per group or tile); one
+pulse is also checked against the FP64 oracle at 1e-10.
 Graph replays are interleaved with host downloads and eager work, and every
 replay's result is compared.
 """
@@ -51,26 +50,27 @@ def _close(props):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("mirror_R", [0.3, 0.0])
-def test_batch_is_bitwise_per_pulse(gpu_lib, prec, mirror_R):
+@pytest.mark.parametrize("var", [(1, 2, 3), (0, 18, 19), (0, 2, 27)])
+def test_batch_is_bitwise_per_pulse(gpu_lib, prec, mirror_R, var):
     """one call over all passes, with and without graphs, against per-pulse runs"""
     nsteps = S * PASSES
     cases = _inputs(mirror_R)
-    print(f"case batch one-shot prec={prec} mirror_R={mirror_R}", flush=True)
-    # per-pulse runs: per-line-group t-line, x/y plans 2 and 3
+    print(f"case batch one-shot prec={prec} mirror_R={mirror_R} variants={var}", flush=True)
+    # per-pulse runs with the variants under test
     ref = []
-    for p in _props(cases, prec, (1, 2, 3)):
+    for p in _props(cases, prec, var):
         p.run(0, nsteps)
         ref.append(p.download())
         assert np.isfinite(ref[-1]).all()
         d = p.diagnostics()
         p.close()
     for graph in (False, True):
-        props = _props(cases, prec, (0, 18, 19))    # the batch takes plan and order, not mode
+        props = _props(cases, prec, var)
         with Batch(props) as b:
             b.run(0, nsteps, graph=graph)
             for i, p in enumerate(props):
                 got = p.download()
-                assert np.array_equal(got, ref[i]), (prec, mirror_R, graph, i)
+                assert np.array_equal(got, ref[i]), (prec, mirror_R, var, graph, i)
             if graph:
                 print(f"  graphs captured: {b.graphs()}", flush=True)
                 assert 1 <= b.graphs() <= PASSES
@@ -81,14 +81,15 @@ def test_batch_is_bitwise_per_pulse(gpu_lib, prec, mirror_R):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-4)])
-def test_batch_pass_by_pass_graph_replays(gpu_lib, prec, tol):
+@pytest.mark.parametrize("var", [(0, 18, 19), (1, 2, 3)])
+def test_batch_pass_by_pass_graph_replays(gpu_lib, prec, tol, var):
     """one call per pass: interior passes replay the same graph; every replay
     is compared with per-pulse runs split the same way, and pulse 0 with the
     oracle"""
     cases = _inputs(0.3)
-    print(f"case batch pass-by-pass prec={prec}", flush=True)
-    ref_props = _props(cases, prec, (1, 2, 3))
-    props = _props(cases, prec, (0, 2, 3))
+    print(f"case batch pass-by-pass prec={prec} variants={var}", flush=True)
+    ref_props = _props(cases, prec, var)
+    props = _props(cases, prec, var)
     oracle_A = cases[0][1]
     with Batch(props) as b:
         for k in range(PASSES):
@@ -98,7 +99,7 @@ def test_batch_pass_by_pass_graph_replays(gpu_lib, prec, tol):
             oracle_A, _ = O.run_fixed(cases[0][0], oracle_A, k * S, S)
             for i, p in enumerate(props):
                 got = p.download()          # host work between replays
-                assert np.array_equal(got, ref_props[i].download()), (prec, k, i)
+                assert np.array_equal(got, ref_props[i].download()), (prec, var, k, i)
             err = O.rel_l2(props[0].download(), oracle_A)
             print(f"  pass {k}: graphs={b.graphs()} pulse-0 rel_l2 vs oracle {err:.3e}", flush=True)
             assert err <= tol
