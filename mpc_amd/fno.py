@@ -124,6 +124,7 @@ class Ops:
         if lib().mpc_sizeof_pw() != C.sizeof(MpcPw):
             raise RuntimeError("mpc_pw layout mismatch between library and binding")
         self.ws = DArr((1 << 22,))                # split-K workspace (16 MiB)
+        self.gemm_flops = 0                       # 2 M N K summed over the matrix-core products issued
 
     def gemm(self, M, N, K, A, rsA, csA, B, rsB, csB, Cm, rsC, csC, alpha=1.0, beta=0.0,
              b1=(1, 0, 0, 0), b2=(1, 0, 0, 0), offA=0, offB=0, offC=0):
@@ -144,6 +145,7 @@ class Ops:
             ks = min(max(1, 2048 // tiles), max(1, K // 1024), max(1, self.ws.numel() // (M * N * b1[0] * b2[0])))
         g.ksplit = ks
         g.W = _ptr(self.ws) if ks > 1 else None
+        self.gemm_flops += 2 * M * N * K * b1[0] * b2[0]
         check(lib().mpc_gemm_bf16(C.cast(C.pointer(g), C.c_void_p), _stream()), "mpc_gemm_bf16")
 
     def combine(self, src, dst, nslice, m, nb, h, sign, bwd=False):
@@ -502,7 +504,15 @@ def time_network(batch: int = 1, reps: int = 20, train_steps: int = 0, seed: int
         net.forward(x)
         sync()
         ts.append(time.perf_counter() - t0)
-    out = {"batch": batch, "forward_ms": 1e3 * float(np.median(ts))}
+    f0 = net.ops.gemm_flops
+    net.forward(x)
+    sync()
+    fwd_flops = net.ops.gemm_flops - f0
+    fwd_s = float(np.median(ts))
+    # the BF16 matrix-core share of the forward: its GEMM flops over the whole
+    # forward time, against the dense BF16 peak (2.5 PFLOP/s)
+    out = {"batch": batch, "forward_ms": 1e3 * fwd_s, "forward_gemm_gflop": fwd_flops / 1e9,
+           "forward_bf16_tflops": fwd_flops / fwd_s / 1e12, "bf16_peak_frac": fwd_flops / fwd_s / 2.5e15}
     if train_steps:
         tgt = DArr.of(rng.standard_normal((batch, cfg.cout, n)).astype(np.float32) * 0.1)
         net.loss_and_grad(x, tgt)

@@ -8,6 +8,9 @@
   the plain pass, so the assisted trajectory is the plain one (up to the
   float round trip of export / import, 1e-5); with checks that always pass,
   the network takes every pass but the plain_every-th.
+* Training gain: 300 Adam steps on 4 solver pairs bring the network's error
+  on them below 0.7 of the linear pass map's (the method without the
+  network).
 """
 import math
 
@@ -79,3 +82,21 @@ def test_passed_checks_keep_the_network_pass(gpu_lib):
     assert r["kinds"] == ["net", "net", "plain"] * 2
     assert all(np.isfinite(r["errs"]))
     assert r["net_forward_ms"] is not None and r["net_forward_ms"] > 0
+
+
+def test_training_beats_the_linear_map_on_its_pairs(gpu_lib):
+    """a few hundred Adam steps on a handful of solver pairs bring the
+    network's error on those pairs below the linear pass map's (the method
+    without the network); generalisation to unseen pulses is measured by
+    bench.py --surrogate, not here"""
+    print("case surrogate training gain", flush=True)
+    data = surrogate.generate(2, 2, seed=13, log=lambda *a: None)
+    ident, linear = surrogate.baseline_errors(data, [0, 1])
+    tr = surrogate.Trainer(FnoConfig(), data, batch=4, seed=2)
+    first = tr.eval_loss([0, 1])
+    for i in range(300):
+        tr.step(2e-3 * (1.0 - i / 300))
+    last = tr.eval_loss([0, 1])
+    print(f"  pair rel L2: no change {ident:.3e}, linear map {linear:.3e}, network {first:.3e} -> {last:.3e}",
+          flush=True)
+    assert last < 0.7 * linear        # measured: 0.51x after 300 steps
