@@ -4,6 +4,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
     python bench.py --batch        (the 48-pulse batched workload instead)
     python bench.py --surrogate    (the pass surrogate experiment instead)
+    python bench.py --adaptive     (adaptive step doubling + Richardson instead)
 
 One "step" is one z-step L(dz/2) N(dz) L(dz/2) of the whole 256x256x2048 field
 (Kerr + self-steepening + ADK/avalanche plasma + diffraction + dispersion),
@@ -173,6 +174,39 @@ def load_traffic(prec):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def adaptive_bench(steps, warmup):
+    """adaptive step doubling + Richardson at 256x256x2048 (FP32 and FP64):
+    each attempt is one 2dz step (C1) and two dz steps (C2), i.e. three
+    isolated split steps, plus the error norm, the beam-width cap and the
+    Richardson combination; an accepted step advances z by 2dz"""
+    from mpc_amd import Propagator, configs, make_inputs
+    wl = configs.cfg2_production_256()
+    inp = make_inputs(wl)
+    out = {}
+    for prec in ("f32", "f64"):
+        p = Propagator(inp, prec)
+        p.upload(inp.A)
+        p.calibrate()
+        p.autotune()
+        r = p.run_adaptive(0.0, inp.params.dzz, 1, warmup)
+        t0 = time.perf_counter()
+        r2 = p.run_adaptive(r["z"], r["dzz"], r["signmem"], steps)
+        dt = time.perf_counter() - t0
+        n = r2["accepted"]
+        out[prec] = {"accepted_steps_per_s": round(n / dt, 3), "accepted": n, "seconds": round(dt, 4),
+                     "z_advanced_m": r2["z"] - r["z"], "dzz_end": r2["dzz"],
+                     "isolated_split_steps_per_s_lower_bound": round(3 * n / dt, 3)}
+        log(f"[bench] adaptive {prec}: {n} accepted steps in {dt:.3f}s ({n / dt:.1f}/s), "
+            f"z {r['z']:.4f} -> {r2['z']:.4f} m")
+        p.close()
+    return {"metric": "adaptive split-step (step doubling + Richardson), accepted steps/s at 256x256x2048",
+            "value": out["f32"]["accepted_steps_per_s"], "unit": "accepted steps/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic (seeded Gaussian pulse + 1e-6 complex noise, argon 2 bar)",
+            "config": {"workload": "256x256x2048 adaptive mode, def_err 1e-3, start dzz = 1 mm",
+                       "per_precision": out}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +216,8 @@ def main(argv=None):
     ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
     ap.add_argument("--batch", action="store_true",
                     help="instead: the 48-pulse batched workload (FP32), batched vs one by one")
+    ap.add_argument("--adaptive", action="store_true",
+                    help="instead: adaptive mode (step doubling + Richardson), accepted steps/s, FP32 and FP64")
     ap.add_argument("--surrogate", action="store_true",
                     help="instead: the pass-surrogate experiment (pairs from the solver, training on a time "
                          "budget, held-out errors, assisted rollouts against plain ones)")
@@ -204,6 +240,9 @@ def main(argv=None):
         return 1
     if a.batch:
         print(json.dumps(batch_bench(a.steps, a.warmup)))
+        return 0
+    if a.adaptive:
+        print(json.dumps(adaptive_bench(a.steps, a.warmup)))
         return 0
     if a.surrogate:
         from mpc_amd import surrogate

@@ -765,6 +765,34 @@ template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *s
     return run_xy<T>(c, s * dzz / 2, false);
 }
 
+/* two consecutive isolated steps of dzz at z and z + dzz, with the middle
+ * half steps merged (their linear factors commute): x y | t | x y (+P on the
+ * t-spectral field) | t | x y, 8 sweeps instead of 10.  signmem is updated
+ * as two step_once calls would. */
+template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem) {
+    int s[2];
+    for (int i = 0; i < 2; i++) {
+        const double zi = z + i * dzz;
+        if (c->p.mirror_R > 0.0) {
+            s[i] = 1;
+            if (signmem) *signmem = 1;
+        } else {
+            s[i] = pass_sign(c, zi);
+            if (signmem && -*signmem == s[i]) *signmem = -*signmem;
+        }
+    }
+    /* every table carries one 1/nt, one per forward/inverse t-FFT pair: the
+     * pair that spans the two steps gets the merged P(dz/2) Q(dz/2) = P(dz) */
+    const cx<T> *P = get_table<T>(c, 't', dzz / 2);
+    const cx<T> *P2 = get_table<T>(c, 't', dzz);
+    if (!P || !P2) return MPC_ENOMEM;
+    RCHK(run_xy<T>(c, s[0] * dzz / 2, false));
+    RCHK(sweep_t<T>(c, P, nullptr, false, true, false, dzz));           /* leaves the field t-spectral */
+    RCHK(run_xy<T>(c, (s[0] + s[1]) * dzz / 2, false, P2));            /* x P(dz) on the y sweep */
+    RCHK(sweep_t<T>(c, nullptr, P, true, true, true, dzz));
+    return run_xy<T>(c, s[1] * dzz / 2, false);
+}
+
 extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
     if (!c) return MPC_EINVAL;
     if (c->domain_freq) return MPC_EINVAL;
@@ -969,8 +997,7 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
         c->field = c->aux1;
         int rc = step_once<T>(c, 2 * dz, *z, &sm1);
         c->field = c->aux2;
-        if (!rc) rc = step_once<T>(c, dz, *z, signmem);
-        if (!rc) rc = step_once<T>(c, dz, *z + dz, signmem);   /* corrected second sub-step position */
+        if (!rc) rc = two_steps<T>(c, dz, *z, signmem);          /* sub-steps at z and z + dz */
         if (!rc) {
             /* BP and calc1oes of C2 for the step-size cap (propagation.c:343-362) */
             mpc_diag dg{};
