@@ -224,6 +224,8 @@ def main(argv=None):
     ap.add_argument("--surrogate-pulses", type=int, default=512, help=argparse.SUPPRESS)
     ap.add_argument("--surrogate-passes", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--train-seconds", type=float, default=240.0, help=argparse.SUPPRESS)
+    ap.add_argument("--surrogate-width", type=int, default=16, help=argparse.SUPPRESS)
+    ap.add_argument("--surrogate-layers", type=int, default=4, help=argparse.SUPPRESS)
     ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
                     help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
@@ -246,8 +248,10 @@ def main(argv=None):
         return 0
     if a.surrogate:
         from mpc_amd import surrogate
+        from mpc_amd.fno import FnoConfig
+        cfg = FnoConfig(width=a.surrogate_width, layers=a.surrogate_layers, hidden=2 * a.surrogate_width)
         r = surrogate.run_experiment(n_pulses=a.surrogate_pulses, passes=a.surrogate_passes,
-                                     train_seconds=a.train_seconds, log=log)
+                                     train_seconds=a.train_seconds, cfg=cfg, log=log)
         print(json.dumps({"metric": "surrogate-assisted passes/s at 64x64x256 (rel L2 per pass vs plain FP32)",
                           "value": r["passes_per_s_assisted"], "unit": "passes/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "bf16 (network), f32 (solver)",

@@ -215,12 +215,12 @@ class FnoConfig:
     X: int = 64
     Y: int = 64
     T: int = 256
-    width: int = 32
+    width: int = 16
     layers: int = 4
     mx: int = 16          # x modes: mx/2 lowest positive and mx/2 negative
     my: int = 16
     mt: int = 32          # rfft modes 0 .. mt-1 along t
-    hidden: int = 64      # projection width
+    hidden: int = 32      # projection width
     cin: int = 4          # Re, Im, energy, pressure
     cout: int = 2
 

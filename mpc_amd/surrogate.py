@@ -198,13 +198,16 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
     x = DArr((1, cfg.cin, n))
     cur = DArr((2, n))
     start = DArr((2, n))
+    sc = scalars_of(wl)
     x.fill(0.0)
     net.forward(x)                               # allocates the network's buffers before any timing
+    for c in range(2):                           # the constant channels; exports only write channels 0-1
+        check(lib().mpc_dev_fill(C.c_void_p(x.ptr + 4 * (2 + c) * n), n, float(sc[c]), None), "mpc_dev_fill")
     sync()
     t_net = []
-    sc = scalars_of(wl)
     errs, kinds = [], []
     t_assist = 0.0
+    t_netpass = []                               # whole network passes: linear map, network, import, checks
     d_cur = None                                 # diagnostics of the current state, when known
     for k in range(passes):
         t0 = time.perf_counter()
@@ -214,8 +217,6 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
             export(p, start, 0, scale)
             linear_pass(p, wl)
             export(p, x, 0, scale)
-            for c in range(2):
-                check(lib().mpc_dev_fill(C.c_void_p(x.ptr + 4 * (2 + c) * n), n, float(sc[c]), None), "mpc_dev_fill")
             sync()
             t1 = time.perf_counter()
             pred = net.forward(x)
@@ -229,6 +230,8 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
             if not (e_ok and w_ok):
                 import_(p, start, 0, scale)      # back to the pass's input, then the plain pass
                 kind = "fallback"
+        if kind == "net":
+            t_netpass.append(time.perf_counter() - t0)
         if kind != "net":
             p.run(k * S, S)
             d_cur = None
@@ -240,7 +243,9 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
     p.close()
     return {"errs": errs, "kinds": kinds, "t_plain": t_plain, "t_assist": t_assist,
             "passes_per_s_plain": passes / t_plain, "passes_per_s_assist": passes / t_assist,
-            "net_forward_ms": 1e3 * float(np.median(t_net)) if t_net else None}
+            "net_forward_ms": 1e3 * float(np.median(t_net)) if t_net else None,
+            "net_pass_ms": 1e3 * float(np.median(t_netpass)) if t_netpass else None,
+            "plain_pass_ms": 1e3 * t_plain / passes}
 
 
 def baseline_errors(data: PairSet, pulses) -> tuple:
@@ -295,7 +300,9 @@ def run_experiment(n_pulses: int = 512, passes: int = 8, train_seconds: float = 
             f"{r['kinds'].count('plain')} plain passes; {r['passes_per_s_assist']:.1f} vs "
             f"{r['passes_per_s_plain']:.1f} passes/s")
     per_pass = [e for r in rolls for e in r["errs"]]
-    return {"pairs": n_pulses * passes, "gen_seconds": round(gen_s, 1),
+    return {"network": {"width": cfg.width, "layers": cfg.layers, "modes": [cfg.mx, cfg.my, cfg.mt],
+                        "hidden": cfg.hidden},
+            "pairs": n_pulses * passes, "gen_seconds": round(gen_s, 1),
             "train_steps": steps, "train_seconds": round(t_train, 1), "batch": batch,
             "final_train_loss": float(np.mean(losses[-100:])) if losses else None,
             "held_out_pair_rel_l2": held_err, "identity_pair_rel_l2": identity_err,
@@ -307,4 +314,10 @@ def run_experiment(n_pulses: int = 512, passes: int = 8, train_seconds: float = 
             "passes_per_s_assisted": float(np.mean([r["passes_per_s_assist"] for r in rolls])),
             "passes_per_s_plain": float(np.mean([r["passes_per_s_plain"] for r in rolls])),
             "net_forward_ms": float(np.median([r["net_forward_ms"] for r in rolls if r["net_forward_ms"]]))
-            if any(r["net_forward_ms"] for r in rolls) else None}
+            if any(r["net_forward_ms"] for r in rolls) else None,
+            "net_pass_ms": float(np.median([r["net_pass_ms"] for r in rolls if r["net_pass_ms"]]))
+            if any(r["net_pass_ms"] for r in rolls) else None,
+            "plain_pass_ms": float(np.mean([r["plain_pass_ms"] for r in rolls])),
+            # with every plain_every-th pass plain, the assisted rate is below
+            # plain_every x the plain rate however fast the network passes are
+            "assisted_speedup_bound": float(plain_every)}
