@@ -41,6 +41,24 @@ def test_run_matches_oracle(gpu_lib, prec, tol, nx, nt, plasm, sst):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+@pytest.mark.parametrize("nx,nt", [(16, 4096), (192, 512), (256, 256), (512, 256)])
+def test_edge_sizes_match_oracle(gpu_lib, prec, tol, nx, nt):
+    """the extremes of the supported line lengths (transverse 16 / 192 (radix
+    3) / 256 / 512, t 4096 and 256), three steps across one mirror"""
+    print(f"case edge size nx={nx} nt={nt} prec={prec}", flush=True)
+    inp, A0 = _case(nx, nt, steps_per_pass=2, passes=2)
+    nsteps = 3
+    ref, _ = O.run_fixed(inp, A0, 0, nsteps)
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, nsteps)
+        got = p.download()
+    err = O.rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e}", flush=True)
+    assert err <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
 def test_single_step_matches_oracle(gpu_lib, prec, tol):
     print(f"case single step prec={prec}", flush=True)
     inp, A0 = _case(32, 512, plasm=1, sst_on=1, mirror_R=0.0)
