@@ -224,7 +224,7 @@ def main(argv=None):
     ap.add_argument("--surrogate-pulses", type=int, default=512, help=argparse.SUPPRESS)
     ap.add_argument("--surrogate-passes", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--train-seconds", type=float, default=240.0, help=argparse.SUPPRESS)
-    ap.add_argument("--surrogate-width", type=int, default=16, help=argparse.SUPPRESS)
+    ap.add_argument("--surrogate-width", type=int, default=32, help=argparse.SUPPRESS)
     ap.add_argument("--surrogate-layers", type=int, default=4, help=argparse.SUPPRESS)
     ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
                     help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")

@@ -92,8 +92,8 @@ def test_training_beats_the_linear_map_on_its_pairs(gpu_lib):
     print("case surrogate training gain", flush=True)
     data = surrogate.generate(2, 2, seed=13, log=lambda *a: None)
     ident, linear = surrogate.baseline_errors(data, [0, 1])
-    # width 32 (the default is 16, which needs more steps to pass the linear
-    # map: about 800 at this learning rate in the full experiment)
+    # width 32, the configured network (width 16 needs more steps to pass the
+    # linear map: about 800 at this learning rate in the full experiment)
     tr = surrogate.Trainer(FnoConfig(width=32, hidden=64), data, batch=4, seed=2)
     first = tr.eval_loss([0, 1])
     for i in range(400):
