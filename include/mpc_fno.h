@@ -6,7 +6,7 @@
  * caller's own) on a HIP stream (void *stream, NULL = default stream) and
  * return 0 or a negative MPC_E* code (mpc.h).  The network built from them (mpc_amd/fno.py) maps the
  * field at one mirror plane to the field after a pass; the plain split step
- * stays the fallback and the judge of its error.
+ * stays the fallback and the measure of its error.
  */
 #ifndef MPC_FNO_H
 #define MPC_FNO_H

@@ -1,5 +1,6 @@
 """Pass surrogate: training data from the solver, training, and surrogate-
-assisted rollouts with the plain split step as fallback and judge.
+assisted rollouts with the plain split step as fallback and as the measure
+of their error.
 
 Workload (the precision-form ranges around the 64x64 workload): pulses on a
 64 x 64 x 256 grid over 4 mm x 2 ps, 0.5 m cell, 200 z-steps per pass,
