@@ -138,6 +138,11 @@ __device__ __forceinline__ double one_minus_exp_neg(double R) {
  * accesses (unit lane stride) and the scan's per-thread segments (lane
  * stride E) are free of bank conflicts */
 template <class G> __device__ __forceinline__ int padp(int i) { return padi<G::PAD>(i); }
+/* the swizzled position of element tid + TPL*m (constant offsets where the
+ * layout allows, see padi_strided) */
+template <class G> __device__ __forceinline__ int padp_tm(int tid, int m) {
+    return padi_strided<G::PAD, G::NT / G::E>(tid, m);
+}
 
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
@@ -233,7 +238,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     __syncthreads();              /* the LDS may still be read by the caller's last exchange */
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const int k = tid + TPL * m;
+        const int k = padp_tm<G>(tid, m);                  /* element tid + TPL*m, swizzled */
         const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
         double I;
         const double iv = plasma_invs(a2, pk, &I);
@@ -241,18 +246,18 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         const double sig = plasma_sigma(iv, pk, &tc);
         const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, iv) * pk.Kon;
         const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
-        U[padp<G>(k)] = 1.0 - a;
-        X[padp<G>(k)] = b * (1.0 - a) + a;
+        U[k] = 1.0 - a;
+        X[k] = b * (1.0 - a) + a;
         if constexpr (L::SV) {
-            S[padp<G>(k)] = sig;
-            V[padp<G>(k)] = iv;
+            S[k] = sig;
+            V[k] = iv;
         } else {
             invs[m] = iv;
         }
         anyc |= (pk.kon2 != 0.0 && pk.ke2 * iv <= ADK_E_MAX);
     }
     auto inv_s = [&](int m) {
-        if constexpr (L::SV) return V[padp<G>(tid + TPL * m)];
+        if constexpr (L::SV) return V[padp_tm<G>(tid, m)];
         else return invs[m];
     };
     __syncthreads();
@@ -264,7 +269,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     if (block_anyc) {
 #pragma unroll
         for (int m = 0; m < E; ++m)
-            X[padp<G>(tid + TPL * m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, inv_s(m)) * pk.Kon;
+            X[padp_tm<G>(tid, m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, inv_s(m)) * pk.Kon;
         __syncthreads();
         /* X <- ions2: m_k = 1 - c_k, c'_k = c_k u_{k+1} */
         affine_suffix_scan<G>(tid, scr,
@@ -278,14 +283,14 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const int k = tid + TPL * m;
-        const double i2 = block_anyc ? X[padp<G>(k)] : 0.0;
-        const double i1 = U[padp<G>(k)] - i2;
+        const int k = padp_tm<G>(tid, m);
+        const double i2 = block_anyc ? X[k] : 0.0;
+        const double i1 = U[k] - i2;
         const double fr = i1 + 2.0 * i2;
         double tc, sig;
         if constexpr (L::SV) {
-            sig = S[padp<G>(k)];
-            tc = pk.kt * V[padp<G>(k)];
+            sig = S[k];
+            tc = pk.kt * V[k];
         } else {
             sig = plasma_sigma(invs[m], pk, &tc);
         }

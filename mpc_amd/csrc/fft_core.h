@@ -165,6 +165,17 @@ template <int PAD> MPC_HD int padi(int i) {
     else if constexpr (PAD < 0) return i ^ ((i >> (-PAD)) & 31);
     else return i;
 }
+/* padi<PAD>(a + STRIDE * m), written so that the m-dependent part becomes a
+ * constant offset (an immediate in the LDS instruction) where the layout
+ * allows it.  XOR swizzle (PAD = -s): the mask reads index bits s..s+4 and
+ * flips bits 0..4, and a multiple of 2^(s+5) touches neither, so
+ * padi(a + c) = padi(a) + c for such c.  Padding (PAD > 0): a multiple c of
+ * PAD gives padi(a + c) = padi(a) + c + c / PAD. */
+template <int PAD, int STRIDE> MPC_HD int padi_strided(int a, int m) {
+    if constexpr (PAD < 0 && STRIDE % (1 << (-PAD + 5)) == 0) return padi<PAD>(a) + STRIDE * m;
+    else if constexpr (PAD > 0 && STRIDE % PAD == 0) return padi<PAD>(a) + (STRIDE + STRIDE / PAD) * m;
+    else return padi<PAD>(a + STRIDE * m);
+}
 /* elements an exchange buffer of n elements occupies under padi<PAD> */
 template <int PAD> constexpr int padded_elems(int n) { return PAD > 0 ? n + n / PAD : n; }
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
@@ -307,7 +318,7 @@ template <class T, int N, int E, int PAD = 0>
 MPC_HD void fft_gather(cx<T> (&v)[E], int tid, const cx<T> *lds, int ls) {
     constexpr int TPL = N / E;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[padi<PAD>(tid + TPL * m) * ls];
+    for (int m = 0; m < E; ++m) v[m] = lds[padi_strided<PAD, TPL>(tid, m) * ls];
 }
 
 /* radix plans: a compile-time list of radices whose product is N */
