@@ -334,8 +334,9 @@ template <class T> static AArgs<T> axis_args(const mpc_ctx *c, int axis, const c
 }
 
 template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, bool mirror,
-                                         const cx<T> *Pt = nullptr) {
-    const AArgs<T> a = axis_args<T>(c, axis, D, mirror, Pt);
+                                         const cx<T> *Pt = nullptr, const void *src = nullptr) {
+    AArgs<T> a = axis_args<T>(c, axis, D, mirror, Pt);
+    a.src = (const cx<T> *)src;
     tic(c, 1 + axis);
     const int rc = launch_axis<T>(a, axis == 0 ? c->nx : c->ny, axis == 0 ? c->ny : c->nx,
                                   axis == 0 ? c->avar : c->avar_y, c->stream);
@@ -345,11 +346,12 @@ template <class T> static int sweep_axis(mpc_ctx *c, int axis, const cx<T> *D, b
 
 /* x then y sweep with the merged diffraction factor; Pt (t-spectral field
  * only) is a per-t factor the y sweep applies before its store */
-template <class T> static int run_xy(mpc_ctx *c, double coef, bool mirror, const cx<T> *Pt = nullptr) {
-    if (coef == 0.0 && !mirror && !Pt) return 0;
+template <class T> static int run_xy(mpc_ctx *c, double coef, bool mirror, const cx<T> *Pt = nullptr,
+                                     const void *src = nullptr) {
+    if (coef == 0.0 && !mirror && !Pt && !src) return 0;
     const cx<T> *D = get_table<T>(c, 'x', coef);
     if (!D) return MPC_ENOMEM;
-    RCHK(sweep_axis<T>(c, 0, D, false));
+    RCHK(sweep_axis<T>(c, 0, D, false, nullptr, src));      /* src: the x sweep reads it, writes c's field */
     return sweep_axis<T>(c, 1, D, mirror, Pt);
 }
 
@@ -749,7 +751,9 @@ extern "C" int mpc_run(mpc_ctx *c, int step0, int nsteps) {
 }
 
 /* one isolated split step (propagation.c:111-372 semantics) */
-template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *signmem) {
+/* src (nullptr: c's field) is the step's input; the step's first sweep reads
+ * it and writes c's field */
+template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *signmem, const void *src = nullptr) {
     int s;
     if (c->p.mirror_R > 0.0) {
         s = 1;
@@ -758,7 +762,7 @@ template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *s
         s = pass_sign(c, z);
         if (signmem && -*signmem == s) *signmem = -*signmem;   /* propagation.c:380-382 */
     }
-    RCHK(run_xy<T>(c, s * dzz / 2, false));
+    RCHK(run_xy<T>(c, s * dzz / 2, false, nullptr, src));
     const cx<T> *P = get_table<T>(c, 't', dzz / 2);
     if (!P) return MPC_ENOMEM;
     RCHK(sweep_t<T>(c, P, P, false, true, true, dzz));
@@ -769,7 +773,7 @@ template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *s
  * half steps merged (their linear factors commute): x y | t | x y (+P on the
  * t-spectral field) | t | x y, 8 sweeps instead of 10.  signmem is updated
  * as two step_once calls would. */
-template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem) {
+template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem, const void *src = nullptr) {
     int s[2];
     for (int i = 0; i < 2; i++) {
         const double zi = z + i * dzz;
@@ -786,7 +790,7 @@ template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *s
     const cx<T> *P = get_table<T>(c, 't', dzz / 2);
     const cx<T> *P2 = get_table<T>(c, 't', dzz);
     if (!P || !P2) return MPC_ENOMEM;
-    RCHK(run_xy<T>(c, s[0] * dzz / 2, false));
+    RCHK(run_xy<T>(c, s[0] * dzz / 2, false, nullptr, src));
     RCHK(sweep_t<T>(c, P, nullptr, false, true, false, dzz));           /* leaves the field t-spectral */
     RCHK(run_xy<T>(c, (s[0] + s[1]) * dzz / 2, false, P2));            /* x P(dz) on the y sweep */
     RCHK(sweep_t<T>(c, nullptr, P, true, true, true, dzz));
@@ -991,13 +995,13 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
     while (*z < c->p.dist && acc < max_steps && guard < 64 * max_steps + 64) {
         guard++;
         const double dz = *dzz;
-        HIPCHK(hipMemcpyAsync(c->aux1, A, bytes, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->aux2, A, bytes, hipMemcpyDeviceToDevice, c->stream));
+        /* C1 and C2 start from A without copies: each one's first sweep reads
+         * A and writes its own buffer */
         int sm1 = *signmem;
         c->field = c->aux1;
-        int rc = step_once<T>(c, 2 * dz, *z, &sm1);
+        int rc = step_once<T>(c, 2 * dz, *z, &sm1, A);
         c->field = c->aux2;
-        if (!rc) rc = two_steps<T>(c, dz, *z, signmem);          /* sub-steps at z and z + dz */
+        if (!rc) rc = two_steps<T>(c, dz, *z, signmem, A);       /* sub-steps at z and z + dz */
         if (!rc) {
             /* BP and calc1oes of C2 for the step-size cap (propagation.c:343-362) */
             mpc_diag dg{};

@@ -3,7 +3,8 @@
  *
  * k_axis: FFT along x or y of a tile of G consecutive t values (layout
  * A[y][x][t], t fastest, so a tile row is G*sizeof(complex) contiguous
- * bytes), x D(k), inverse FFT [x mirror] [x per-t factor].  Replaces the x/y
+ * bytes), x D(k), inverse FFT [x mirror] [x per-t factor], in place or from
+ * a separate source field (src).  Replaces the x/y
  * parts of linearstep (propagation.c:31-57, 77-100); the diffraction operator
  * is separable, so x and y each take one sweep.
  *
@@ -102,11 +103,12 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
         else line_fft<T, N, PL, DIR>(x, tid, lds + gcol, G, a.tw);
     };
 
+    const cx<T> *in = a.src ? a.src : a.field;
     cx<T> nxt[E], pn = mk((T)1, (T)0);
     {
         long outer, t0;
         axis_tile_of(a, lin, outer, t0, G);
-        const cx<T> *b0 = a.field + outer * a.s_outer + t0 + gcol0;
+        const cx<T> *b0 = in + outer * a.s_outer + t0 + gcol0;
 #pragma unroll
         for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
         if (a.Pt) pn = a.Pt[t0 + gcol0];
@@ -130,7 +132,7 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
             if (lnext < ntiles) {
                 long on, tn;
                 axis_tile_of(a, lnext, on, tn, G);
-                const cx<T> *bn = a.field + on * a.s_outer + tn + gcol;
+                const cx<T> *bn = in + on * a.s_outer + tn + gcol;
 #pragma unroll
                 for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * a.s_line];
                 if (a.Pt) pn = a.Pt[tn + gcol];
@@ -216,7 +218,7 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
     {
         long outer, t0;
         axis_tile_of(a, l - tstart[p], outer, t0, G);
-        const cx<T> *b0 = a.field + outer * a.s_outer + t0 + gcol0;
+        const cx<T> *b0 = (a.src ? a.src : a.field) + outer * a.s_outer + t0 + gcol0;
 #pragma unroll
         for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
         if (a.Pt) pn = a.Pt[t0 + gcol0];
@@ -240,7 +242,7 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
             const AArgs<T> &an = d[q];
             long on, tn;
             axis_tile_of(an, ln - tstart[q], on, tn, G);
-            const cx<T> *bn = an.field + on * an.s_outer + tn + gcol;
+            const cx<T> *bn = (an.src ? an.src : an.field) + on * an.s_outer + tn + gcol;
 #pragma unroll
             for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * an.s_line];
             if (an.Pt) pn = an.Pt[tn + gcol];

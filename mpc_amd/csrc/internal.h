@@ -59,6 +59,7 @@ template <class T> struct TArgs {
 
 template <class T> struct AArgs {
     cx<T> *field;
+    const cx<T> *src;    /* read from here instead of field (same layout), or nullptr: in place */
     const cx<T> *tw;     /* n twiddles */
     const cx<T> *D;      /* n-entry propagator (FFT order), includes 1/n */
     const cx<T> *M;      /* mirror table M[y*Nx + x] or nullptr */
