@@ -77,6 +77,19 @@ constexpr double ce_sin2pi(int m, int N) {
 template <int DIR, class T> MPC_HD cx<T> ctw(int m, int N) {
     return mk((T)ce_cos2pi(m, N), (T)(DIR * ce_sin2pi(m, N)));
 }
+/* a * exp(DIR * 2 pi i e / N) for an exponent fixed at compile time (after
+ * unrolling): quarter turns are exact sign swaps, the rest a complex product */
+template <int DIR, class T> MPC_HD cx<T> ctwmul(cx<T> a, int e, int N) {
+    if ((4 * e) % N == 0) {
+        switch (((4 * e) / N) & 3) {
+        case 0: return a;
+        case 1: return rot90<DIR>(a);
+        case 2: return mk(-a.re, -a.im);
+        default: return rot90<-DIR>(a);
+        }
+    }
+    return cmul(a, ctw<DIR, T>(e, N));
+}
 
 /* ---- small DFT codelets, in place, natural order ------------------------- */
 template <int R, int DIR, class T> struct Dft;
@@ -127,7 +140,7 @@ template <int R1, int R2, int DIR, class T> struct DftCT {
             Dft<R1, DIR, T>::run(a);
 #pragma unroll
             for (int k1 = 0; k1 < R1; ++k1)
-                z[n2 * R1 + k1] = (n2 * k1 == 0) ? a[k1] : cmul(a[k1], ctw<DIR, T>(n2 * k1, N));
+                z[n2 * R1 + k1] = ctwmul<DIR, T>(a[k1], n2 * k1, N);
         }
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
