@@ -327,12 +327,8 @@ static __device__ __noinline__ void sincos_big(double x, double *s, double *c) {
 static __device__ __noinline__ void sincos_bigf(float x, float *s, float *c) { sincosf(x, s, c); }
 
 template <class T> __device__ __forceinline__ void sincos_t(T x, T *s, T *c);
-template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) {
-    if (!(fabs(x) <= 1e5)) { sincos_big(x, s, c); return; }
-    const double k = rint(x * 0.63661977236758134308);          /* 2/pi */
-    double r = fma(-k, 1.57079632673412561417e+00, x);            /* pi/2 = h + m + l */
-    r = fma(-k, 6.07710050630396597660e-11, r);
-    r = fma(-k, 2.02226624879595063154e-21, r);
+/* the polynomials on the reduced argument r, |r| <= pi/4 */
+__device__ __forceinline__ void sincos_poly(double r, double *sr_out, double *cr_out) {
     const double r2 = r * r;
     /* sin r = r - r^3/3! + ... ; cos r = 1 - r^2/2! + ... (Horner in r^2) */
     double sp = -1.0 / 121645100408832000.0;                      /* -1/19! */
@@ -344,7 +340,7 @@ template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s
     sp = fma(sp, r2, -1.0 / 5040.0);                              /* -1/7! */
     sp = fma(sp, r2, 1.0 / 120.0);                                /* 1/5! */
     sp = fma(sp, r2, -1.0 / 6.0);                                 /* -1/3! */
-    const double sr = fma(sp * r2, r, r);
+    *sr_out = fma(sp * r2, r, r);
     double cp = 1.0 / 6402373705728000.0;                         /* 1/18! */
     cp = fma(cp, r2, -1.0 / 20922789888000.0);                    /* -1/16! */
     cp = fma(cp, r2, 1.0 / 87178291200.0);                        /* 1/14! */
@@ -354,30 +350,48 @@ template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s
     cp = fma(cp, r2, 1.0 / 720.0);                                /* 1/6! */
     cp = fma(cp, r2, -1.0 / 24.0);                                /* -1/4! */
     cp = fma(cp, r2, 0.5);
-    const double cr = fma(-cp, r2, 1.0);
+    *cr_out = fma(-cp, r2, 1.0);
+}
+template <> __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) {
+    /* a whole wave inside (-pi/4, pi/4) (the usual nonlinear phase of a step)
+     * skips the reduction, which is the identity there (k = 0, r = x, no
+     * quadrant swap): same results, fewer instructions */
+    if (__all(fabs(x) < 0.785)) { sincos_poly(x, s, c); return; }
+    if (!(fabs(x) <= 1e5)) { sincos_big(x, s, c); return; }
+    const double k = rint(x * 0.63661977236758134308);          /* 2/pi */
+    double r = fma(-k, 1.57079632673412561417e+00, x);            /* pi/2 = h + m + l */
+    r = fma(-k, 6.07710050630396597660e-11, r);
+    r = fma(-k, 2.02226624879595063154e-21, r);
+    double sr, cr;
+    sincos_poly(r, &sr, &cr);
     const int q = ((int)k) & 3;
     const double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
     *s = (q & 2) ? -ss : ss;
     *c = ((q + 1) & 2) ? -cc : cc;
 }
-template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) {
-    if (!(fabsf(x) <= 1e5f)) { sincos_bigf(x, s, c); return; }
-    const float k = rintf(x * 0.636619772f);
-    float r = fmaf(-k, 1.5703125f, x);                            /* pi/2 in three parts */
-    r = fmaf(-k, 4.8351287841796875e-4f, r);
-    r = fmaf(-k, 3.13916473e-7f, r);
+__device__ __forceinline__ void sincos_poly(float r, float *sr_out, float *cr_out) {
     const float r2 = r * r;
     float sp = 1.0f / 362880.0f;                                  /* Taylor to r^9 */
     sp = fmaf(sp, r2, -1.0f / 5040.0f);
     sp = fmaf(sp, r2, 1.0f / 120.0f);
     sp = fmaf(sp, r2, -1.0f / 6.0f);
-    const float sr = fmaf(sp * r2, r, r);
+    *sr_out = fmaf(sp * r2, r, r);
     float cp = -1.0f / 3628800.0f;                                /* Taylor to r^10 */
     cp = fmaf(cp, r2, 1.0f / 40320.0f);
     cp = fmaf(cp, r2, -1.0f / 720.0f);
     cp = fmaf(cp, r2, 1.0f / 24.0f);
     cp = fmaf(cp, r2, -0.5f);
-    const float cr = fmaf(cp, r2, 1.0f);
+    *cr_out = fmaf(cp, r2, 1.0f);
+}
+template <> __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) {
+    if (__all(fabsf(x) < 0.785f)) { sincos_poly(x, s, c); return; }   /* as in the FP64 form */
+    if (!(fabsf(x) <= 1e5f)) { sincos_bigf(x, s, c); return; }
+    const float k = rintf(x * 0.636619772f);
+    float r = fmaf(-k, 1.5703125f, x);                            /* pi/2 in three parts */
+    r = fmaf(-k, 4.8351287841796875e-4f, r);
+    r = fmaf(-k, 3.13916473e-7f, r);
+    float sr, cr;
+    sincos_poly(r, &sr, &cr);
     const int q = ((int)k) & 3;
     const float ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
     *s = (q & 2) ? -ss : ss;
