@@ -113,9 +113,19 @@ __device__ __forceinline__ double plasma_sigma(double invs, const PlasmaK &k, do
     /* 1 + K5 tc^2 is in [1, 1e60] for any I >= the 1e-30 floor */
     return k.K4 * tc * fast_rcp(1.0 + k.K5 * (tc * tc));
 }
-/* 1 - exp(-R) = sum_{k>=1} (-1)^(k+1) R^k / k!: twelve terms up to R = 0.1
- * (truncation below 2e-22 relative), -expm1(-R) beyond */
+/* 1 - exp(-R) = sum_{k>=1} (-1)^(k+1) R^k / k!: four terms when a whole
+ * wave has R <= 2^-12, twelve up to R = 0.1 (truncation below 2e-22
+ * relative), -expm1(-R) beyond */
 __device__ __forceinline__ double one_minus_exp_neg(double R) {
+    if (__all(R <= 0x1p-12)) {
+        /* a whole wave of small densities: four terms (truncation below
+         * R^4/120 < 3e-17 relative) */
+        double p = -1.0 / 24.0;
+        p = fma(p, R, 1.0 / 6.0);
+        p = fma(p, R, -0.5);
+        p = fma(p, R, 1.0);
+        return p * R;
+    }
     if (R <= 0.1) {
         double p = -1.0 / 479001600.0;                  /* -1/12! */
         p = fma(p, R, 1.0 / 39916800.0);
