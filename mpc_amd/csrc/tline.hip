@@ -61,11 +61,12 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
         for (int m = 0; m < E; ++m) {
             const T a2 = v[m].re * v[m].re + v[m].im * v[m].im;
             if (a.sst_on) {
-                /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232) */
-                const cx<T> t1 = cmulc(d[m], v[m]);
-                const cx<T> t2 = cmulc(v[m], d[m]);
-                const T sre = a.sst_coef * ((T)2 * t1.re + t2.re);
-                const T sim = a.sst_coef * ((T)2 * t1.im + t2.im);
+                /* S = gamma2/w0 (2 dA conj(A) + A conj(dA))  (propagation.c:232);
+                 * A conj(dA) = conj(dA conj(A)), so S = gamma2/w0 (3 Re t + i Im t)
+                 * with t = dA conj(A) */
+                const cx<T> t = cmulc(d[m], v[m]);
+                const T sre = a.sst_coef * ((T)3 * t.re);
+                const T sim = a.sst_coef * t.im;
                 d[m] = mk(-sre, a.gamma * a2 - sim);
             } else {
                 d[m] = mk((T)0, a.gamma * a2);
