@@ -60,11 +60,11 @@ __device__ __forceinline__ void axis_apply(const AArgs<T> &a, cx<T> (&v)[ACfg<T,
     constexpr int E = C::E, TPL = C::TPL;
     fft(v, std::integral_constant<int, -1>{});
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], PERSIST ? Dc[m] : a.D[tid + TPL * m]);
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], PERSIST ? Dc[m] : gload(a.D, tid + TPL * m));
     fft(v, std::integral_constant<int, +1>{});
     if (a.M) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.M[(long)(tid + TPL * m) * a.m_line + outer * a.m_outer]);
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], gload(a.M, (long)(tid + TPL * m) * a.m_line + outer * a.m_outer));
     }
     if (a.Pt) {
 #pragma unroll
@@ -94,7 +94,7 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
     if constexpr (PERSIST) {
         twc.load(a.tw, tid0);
 #pragma unroll
-        for (int m = 0; m < E; ++m) Dc[m] = a.D[tid0 + TPL * m];
+        for (int m = 0; m < E; ++m) Dc[m] = gload(a.D, tid0 + TPL * m);
     }
     int tid = tid0, gcol = gcol0;
     auto fft = [&](cx<T>(&x)[E], auto dir) {
@@ -110,8 +110,8 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
         axis_tile_of(a, lin, outer, t0, G);
         const cx<T> *b0 = in + outer * a.s_outer + t0 + gcol0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
-        if (a.Pt) pn = a.Pt[t0 + gcol0];
+        for (int m = 0; m < E; ++m) nxt[m] = gload(b0, (long)(tid0 + TPL * m) * a.s_line);
+        if (a.Pt) pn = gload(a.Pt, t0 + gcol0);
     }
     for (;;) {
         /* tid and gcol re-enter as opaque values each iteration, so index
@@ -134,13 +134,13 @@ __device__ __forceinline__ void axis_work(const AArgs<T> &a, long lin, long lstr
                 axis_tile_of(a, lnext, on, tn, G);
                 const cx<T> *bn = in + on * a.s_outer + tn + gcol;
 #pragma unroll
-                for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * a.s_line];
-                if (a.Pt) pn = a.Pt[tn + gcol];
+                for (int m = 0; m < E; ++m) nxt[m] = gload(bn, (long)(tid + TPL * m) * a.s_line);
+                if (a.Pt) pn = gload(a.Pt, tn + gcol);
             }
         }
         axis_apply<T, N, PL, G_, PERSIST>(a, v, p, outer, tid, Dc, fft);
 #pragma unroll
-        for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
+        for (int m = 0; m < E; ++m) gstore(base, (long)(tid + TPL * m) * a.s_line, v[m]);
         if (!PERSIST || lnext >= ntiles) break;
         lin = lnext;
     }
@@ -205,7 +205,7 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
     auto load_field = [&] {
         twc.load(a.tw, tid0);
 #pragma unroll
-        for (int m = 0; m < E; ++m) Dc[m] = a.D[tid0 + TPL * m];
+        for (int m = 0; m < E; ++m) Dc[m] = gload(a.D, tid0 + TPL * m);
     };
     load_field();
     int tid = tid0, gcol = gcol0;
@@ -220,8 +220,8 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
         axis_tile_of(a, l - tstart[p], outer, t0, G);
         const cx<T> *b0 = (a.src ? a.src : a.field) + outer * a.s_outer + t0 + gcol0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) nxt[m] = b0[(long)(tid0 + TPL * m) * a.s_line];
-        if (a.Pt) pn = a.Pt[t0 + gcol0];
+        for (int m = 0; m < E; ++m) nxt[m] = gload(b0, (long)(tid0 + TPL * m) * a.s_line);
+        if (a.Pt) pn = gload(a.Pt, t0 + gcol0);
     }
     for (;;) {
         tid = tid0;
@@ -244,12 +244,12 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
             axis_tile_of(an, ln - tstart[q], on, tn, G);
             const cx<T> *bn = (an.src ? an.src : an.field) + on * an.s_outer + tn + gcol;
 #pragma unroll
-            for (int m = 0; m < E; ++m) nxt[m] = bn[(long)(tid + TPL * m) * an.s_line];
-            if (an.Pt) pn = an.Pt[tn + gcol];
+            for (int m = 0; m < E; ++m) nxt[m] = gload(bn, (long)(tid + TPL * m) * an.s_line);
+            if (an.Pt) pn = gload(an.Pt, tn + gcol);
         }
         axis_apply<T, N, PL, G_, true>(a, v, pf, outer, tid, Dc, fft);
 #pragma unroll
-        for (int m = 0; m < E; ++m) base[(long)(tid + TPL * m) * a.s_line] = v[m];
+        for (int m = 0; m < E; ++m) gstore(base, (long)(tid + TPL * m) * a.s_line, v[m]);
         if (ln >= total) break;
         if (q != p) {
             /* the next field's tables; the LDS of the last tile is read by

@@ -14,6 +14,30 @@
 
 namespace mpc {
 
+/*
+ * Loads and stores through global-address-space pointers.  A kernel that
+ * reads its pointers from device memory (the batched sweeps take them from an
+ * argument array) cannot tell global from LDS or private addresses and would
+ * access through flat instructions.  Flat accesses also count against the LDS
+ * wait counter, so every LDS wait of a transform would wait for the prefetch
+ * of the next line as well.  All pointers given here address global memory.
+ */
+template <class T> __device__ __forceinline__ T gload(const T *p, long i) {
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+template <class T> __device__ __forceinline__ cx<T> gload(const cx<T> *p, long i) {
+    typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    const v2 r = ((const __attribute__((address_space(1))) v2 *)p)[i];
+    return mk(r.x, r.y);
+}
+template <class T> __device__ __forceinline__ void gstore(cx<T> *p, long i, cx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    v2 r;
+    r.x = v.re;
+    r.y = v.im;
+    ((__attribute__((address_space(1))) v2 *)p)[i] = r;
+}
+
 /* ---------------------------------------------------------------------- */
 /* FFT driver: unrolls Plan<N>::R into stages with LDS exchange between    */
 /* ---------------------------------------------------------------------- */

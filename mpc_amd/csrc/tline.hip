@@ -42,7 +42,7 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
     if (!a.in_freq) fft(v, Fwd{});
     if (a.P) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.P[tid + TPL * m]);
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], gload(a.P, tid + TPL * m));
     }
     prefetch();
 
@@ -51,7 +51,7 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
         if (a.sst_on) {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                const T w = PERSIST ? wk[m] : a.wfft[tid + TPL * m];
+                const T w = PERSIST ? wk[m] : gload(a.wfft, tid + TPL * m);
                 d[m] = mk(w * v[m].im, -w * v[m].re);   /* -i omega A(omega) */
             }
             fft(d, Inv{});                             /* dA */
@@ -95,7 +95,7 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
     }
     if (a.Q) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.Q[tid + TPL * m]);
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], gload(a.Q, tid + TPL * m));
     }
     if (a.out_time) fft(v, Inv{});
 }
@@ -138,7 +138,7 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
         twc.load(a.tw, tid);
         if (a.do_nl && a.sst_on) {
 #pragma unroll
-            for (int m = 0; m < E; ++m) wk[m] = a.wfft[tid + TPL * m];
+            for (int m = 0; m < E; ++m) wk[m] = gload(a.wfft, tid + TPL * m);
         }
     }
     unsigned char *lraw;
@@ -153,7 +153,7 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
     {
         const cx<T> *g0 = a.field + (grp * LPB + lslot) * NT;
 #pragma unroll
-        for (int m = 0; m < E; ++m) nxt[m] = g0[tid + TPL * m];
+        for (int m = 0; m < E; ++m) nxt[m] = gload(g0, tid + TPL * m);
     }
     double rmax = 0.0;
     for (;;) {
@@ -176,12 +176,12 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
                 if (gnext < ngroups) {
                     const cx<T> *gn = a.field + (gnext * LPB + lslot) * NT;
 #pragma unroll
-                    for (int m = 0; m < E; ++m) nxt[m] = gn[tid + TPL * m];
+                    for (int m = 0; m < E; ++m) nxt[m] = gload(gn, tid + TPL * m);
                 }
             }
         });
 #pragma unroll
-        for (int m = 0; m < E; ++m) g[tid + TPL * m] = v[m];
+        for (int m = 0; m < E; ++m) gstore(g, tid + TPL * m, v[m]);
         if (!PERSIST || gnext >= ngroups) break;
         grp = gnext;
     }
@@ -263,7 +263,7 @@ __device__ __forceinline__ void tline_work_batch(const TArgs<T> *__restrict__ d,
         twc.load(a.tw, tid);
         if (a.do_nl && a.sst_on) {
 #pragma unroll
-            for (int m = 0; m < E; ++m) wk[m] = a.wfft[tid + TPL * m];
+            for (int m = 0; m < E; ++m) wk[m] = gload(a.wfft, tid + TPL * m);
         }
     };
     load_field();
@@ -278,7 +278,7 @@ __device__ __forceinline__ void tline_work_batch(const TArgs<T> *__restrict__ d,
     {
         const cx<T> *g0 = a.field + ((g - gstart[p]) * LPB + lslot) * NT;
 #pragma unroll
-        for (int m = 0; m < E; ++m) nxt[m] = g0[tid + TPL * m];
+        for (int m = 0; m < E; ++m) nxt[m] = gload(g0, tid + TPL * m);
     }
     double rmax = 0.0;
     for (;;) {
@@ -300,11 +300,11 @@ __device__ __forceinline__ void tline_work_batch(const TArgs<T> *__restrict__ d,
                 const cx<T> *fn = pn == p ? a.field : d[pn].field;
                 const cx<T> *src = fn + ((gn - gstart[pn]) * LPB + lslot) * NT;
 #pragma unroll
-                for (int m = 0; m < E; ++m) nxt[m] = src[tid + TPL * m];
+                for (int m = 0; m < E; ++m) nxt[m] = gload(src, tid + TPL * m);
             }
         });
 #pragma unroll
-        for (int m = 0; m < E; ++m) gp[tid + TPL * m] = v[m];
+        for (int m = 0; m < E; ++m) gstore(gp, tid + TPL * m, v[m]);
         if (gn >= total || pn != p) {
             /* leaving field p */
             if constexpr (PLASMA) {
