@@ -114,8 +114,8 @@ def cpu_baseline(inp, max_seconds=30.0):
 
 def batch_bench(steps, warmup):
     """the 48-pulse batched workload in FP32: every pulse advances `steps`
-    z-steps in one batched call (one launch per sweep and size class, a HIP
-    graph per pass), against the same pulses run one after another.  The rate
+    z-steps in one batched call (one launch per sweep and group of same-size
+    pulses, a HIP graph per pass), against the same pulses run one after another.  The rate
     is in 256x256x2048-equivalent steps/s: sum_p N_p * steps / N_ref per s.
     `steps` must stay below the 300 steps of a pass (no mirror inside)."""
     from mpc_amd import Batch, Propagator, configs, make_inputs

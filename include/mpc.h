@@ -130,14 +130,18 @@ int  mpc_variant_counts(int *tline_variants, int *axis_variants);
 /*
  * Batched runs (no counterpart in the reference): pulses of uneven grids,
  * energies and media, one context each (same precision, steps_per_pass and
- * npasses), advance together.  Every sweep of a step is one launch per size
- * class covering all pulses of that class; with use_graph the launches of
- * each pass are captured once into a HIP graph and replayed for every pass
- * with the same launch list.  Each launch takes its class's first pulse's
- * kernel variants (mpc_set_variants_xy / mpc_autotune): the persistent
- * variants walk the line groups or tiles of all pulses of the class with the
- * workgroups that fit on the device at once, the others take one workgroup
- * per group or tile.  Each pulse's result is bit-identical to mpc_run on its
+ * npasses), advance together.  Pulses of one size class are split into
+ * launch groups of at most 256 MB of fields, and every sweep of a step is one
+ * launch per group.  Within a pass each group runs all its sweeps before the
+ * next group starts, so a pulse's sweep finds part of what its previous
+ * sweep wrote still in the last-level cache (when the sweep kinds group the
+ * pulses differently, all groups' launches of one sweep run first instead).
+ * With use_graph the launches of each pass are captured once into a HIP
+ * graph and replayed for every pass with the same launch list.  Each launch
+ * takes its group's first pulse's kernel variants (mpc_set_variants_xy /
+ * mpc_autotune): the persistent variants walk the line groups or tiles of
+ * all pulses of the group with the workgroups that fit on the device at
+ * once, the others take one workgroup per group or tile.  Each pulse's result is bit-identical to mpc_run on its
  * context with those variants.  The batch does not own the contexts; destroy
  * it before them.
  */

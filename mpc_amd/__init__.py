@@ -194,8 +194,9 @@ class Propagator:
 
 class Batch:
     """Pulses advancing together (mpc_batch): every sweep of a step is one
-    launch per size class, and with graph=True each pass's launches are a
-    HIP graph captured once and replayed.  The propagators stay owned by the
+    launch per group of same-size pulses (at most 256 MB of fields per
+    group), and with graph=True each pass's launches are a HIP graph
+    captured once and replayed.  The propagators stay owned by the
     caller; close the batch first."""
 
     def __init__(self, props: list):

@@ -526,6 +526,11 @@ struct BLaunch {
 };
 }
 
+/* pulses of one size class share a launch up to this many field bytes (the
+ * last-level cache, 256 MB on MI355X): a group's sweeps can then run back to
+ * back and each finds part of what the previous one wrote still cached */
+constexpr size_t BATCH_GROUP_BYTES = (size_t)256 << 20;
+
 /* a device copy of bytes, shared by identical contents */
 static const void *pool_put(mpc_batch *b, const void *src, size_t bytes) {
     std::string key((const char *)src, bytes);
@@ -552,7 +557,23 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
         auto f = std::find_if(cls.begin(), cls.end(), [&](const Cls &k) { return k.len == len && k.plasma == pl; });
         if (f == cls.end()) { cls.push_back({len, pl, {p}}); } else f->m.push_back(p);
     }
+    /* each class in groups of at most BATCH_GROUP_BYTES of fields (at least
+     * one field per group) */
+    std::vector<Cls> grp;
     for (const Cls &k : cls) {
+        size_t bytes = 0;
+        for (size_t j = 0; j < k.m.size(); j++) {
+            const mpc_ctx *c = b->ctx[k.m[j]];
+            const size_t fb = c->esz * (size_t)c->N;
+            if (j == 0 || bytes + fb > BATCH_GROUP_BYTES) {
+                grp.push_back({k.len, k.plasma, {}});
+                bytes = 0;
+            }
+            grp.back().m.push_back(k.m[j]);
+            bytes += fb;
+        }
+    }
+    for (const Cls &k : grp) {
         BLaunch L{};
         L.kind = kind; L.len = k.len; L.plasma = k.plasma; L.n = (int)k.m.size(); L.step = plans[0][i].step;
         L.members = k.m;
@@ -613,6 +634,21 @@ template <class T> static int batch_launch(mpc_batch *b, const BLaunch &L) {
     return launch_axis_batch<T>((const AArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.variant, b->stream);
 }
 
+/* launches [i0, i1) of one pass in group-major order: each group's launches
+ * in their own order, one group after another.  Applied only when every sweep
+ * kind splits the pulses into the same groups (each pulse's launches then
+ * stay in order); otherwise the sweep-major order is kept. */
+static void group_major(std::vector<BLaunch> &ls, size_t i0, size_t i1, size_t np) {
+    std::vector<const std::vector<int> *> owner(np, nullptr);
+    for (size_t i = i0; i < i1; i++)
+        for (int p : ls[i].members) {
+            if (!owner[p]) owner[p] = &ls[i].members;
+            else if (*owner[p] != ls[i].members) return;
+        }
+    std::stable_sort(ls.begin() + (long)i0, ls.begin() + (long)i1,
+                     [](const BLaunch &a, const BLaunch &b) { return a.members[0] < b.members[0]; });
+}
+
 /* the launch list of a pass, as a graph key */
 static std::string launch_key(const std::vector<BLaunch> &ls, size_t i0, size_t i1) {
     std::string k;
@@ -641,6 +677,12 @@ template <class T> static int batch_run(mpc_batch *b, int step0, int nsteps, boo
     /* every table and descriptor exists before anything is launched or captured */
     std::vector<BLaunch> ls;
     for (size_t i = 0; i < plans[0].size(); i++) RCHK(batch_group<T>(b, plans, i, ls));
+    for (size_t j0 = 0; j0 < ls.size();) {
+        size_t j1 = j0;
+        while (j1 < ls.size() && ls[j1].step / b->S == ls[j0].step / b->S) j1++;
+        group_major(ls, j0, j1, np);
+        j0 = j1;
+    }
     for (mpc_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
     size_t i0 = 0;
     while (i0 < ls.size()) {
