@@ -1,5 +1,5 @@
-"""GPU: batched runs (many pulses, one launch per sweep and size class, a HIP
-graph per pass) against per-pulse runs and the oracle.
+"""GPU: batched runs (many pulses, one launch per sweep and launch group of
+same-size pulses, a HIP graph per pass) against per-pulse runs and the oracle.
 
 Bar: each pulse's batched result is bit-identical to mpc_run on its own
 context with the same kernel variants (the batch takes each size class's
@@ -108,6 +108,41 @@ def test_batch_pass_by_pass_graph_replays(gpu_lib, prec, tol, var):
         assert b.graphs() <= 2
     _close(props)
     _close(ref_props)
+
+
+# every sweep kind groups these pulses the same way, so each launch group runs
+# its sweeps back to back; the three 128^2 x 1024 fields (134 MB each in FP32,
+# 268 MB in FP64) also split their class into several launch groups
+GROUPED_SPECS = [dict(nx=128, nt=1024, energy=1e-3, pressure=2.0, seed=21),
+                 dict(nx=32, nt=512, energy=3e-4, pressure=1.0, seed=22),
+                 dict(nx=128, nt=1024, energy=2e-4, pressure=2.5, seed=23, waist=0.3e-3),
+                 dict(nx=128, nt=1024, energy=5e-4, pressure=0.5, seed=24)]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("var", [(0, 18, 19), (1, 2, 3)])
+def test_group_major_batch_is_bitwise_per_pulse(gpu_lib, prec, var):
+    """launch groups run one after another within a pass (the launch order
+    differs from the sweep-major one), with the mirror inside: each pulse
+    still equals its own per-pulse run bit for bit"""
+    nsteps = S * PASSES
+    cases = []
+    for s in GROUPED_SPECS:
+        inp = make_inputs(configs.small(steps_per_pass=S, passes=PASSES, mirror_R=0.3, **s))
+        cases.append((inp, inp.calibrated()))
+    print(f"case batch group-major prec={prec} variants={var}", flush=True)
+    ref = []
+    for p in _props(cases, prec, var):
+        p.run(0, nsteps)
+        ref.append(p.download())
+        p.close()
+    for graph in (False, True):
+        props = _props(cases, prec, var)
+        with Batch(props) as b:
+            b.run(0, nsteps, graph=graph)
+            for i, p in enumerate(props):
+                assert np.array_equal(p.download(), ref[i]), (prec, var, graph, i)
+        _close(props)
 
 
 def test_batch_rejects_mixed_schedules(gpu_lib):
