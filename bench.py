@@ -174,6 +174,23 @@ def load_traffic(prec):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def load_valu(prec, variant):
+    """VALU wave-instructions per launch of the t-line variant in use (from the
+    SQ-counter run) and the issue floor they set: one wave-instruction per
+    SIMD every 4 cycles on 1024 SIMDs at 2.4 GHz"""
+    path = os.path.join(ROOT, "profiles", f"valu_tline_256x256x2048_{prec}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    v = d["by_variant"].get(str(variant))
+    if v is None:
+        return None
+    floor_ms = v["valu_instr"] * d["cycles_per_instr"] / (d["simds"] * d["clock_ghz"] * 1e9) * 1e3
+    return {"kernel": v["kernel"], "valu_instr_per_launch": v["valu_instr"], "issue_floor_ms": round(floor_ms, 4),
+            "source": d["source"]}
+
+
 def adaptive_bench(steps, warmup):
     """adaptive step doubling + Richardson at 256x256x2048 (FP32 and FP64):
     each attempt is one 2dz step (C1) and two dz steps (C2), i.e. three
@@ -291,6 +308,12 @@ def main(argv=None):
             "kernel": KERNELS[dom], "kernel_ms": round(kms32[dom], 4),
             "all_kernels_ms": {KERNELS[i]: round(kms32[i], 4) for i in range(3)},
             "algorithmic_bytes_per_launch": per_launch_bytes}
+    # the t-line is bound by its vector ALUs, not by HBM: its VALU issue floor
+    # against the measured launch time
+    valu = load_valu(dtype, var32[0]) if dom == 0 else None
+    if valu is not None:
+        valu["frac"] = round(valu["issue_floor_ms"] / kms32[dom], 4)
+        roof["valu"] = valu
 
     cpu = None
     if not a.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
