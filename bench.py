@@ -89,9 +89,21 @@ def timed_run(inp, prec, warmup, steps, with_kernel_timing=True):
     return steps / dt, dt, A, kms, bytes_field, variants
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(inp, max_seconds=30.0):
     """FP64 CPU oracle (test infrastructure) on the same workload: 1 warm-up
-    step, then steps until ~10 s have elapsed (at most 3)."""
+    step, then steps until ~10 s have elapsed (at most 3).  Returns the
+    baseline record, the oracle's field after those steps and their count."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     O.oracle()
@@ -101,15 +113,40 @@ def cpu_baseline(inp, max_seconds=30.0):
     A, _ = O.run_fixed(inp, A, 0, 1)
     log(f"[bench] cpu oracle warm-up step {time.perf_counter() - t0:.1f}s on {threads} threads")
     n, t0 = 0, time.perf_counter()
+    rho = 0.0
     while n < 3:
-        A, _ = O.run_fixed(inp, A, 1 + n, 1)
+        A, rho = O.run_fixed(inp, A, 1 + n, 1)
         n += 1
         if time.perf_counter() - t0 > min(10.0, max_seconds):
             break
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} z-step(s) of the full 256x256x2048 workload after 1 warm-up step, "
-                      f"FP64 oracle (oracle/, OpenMP {threads} threads), {dt:.1f} s"}
+    rec = {"value": n / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+           "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+           "sample": f"{n} z-step(s) of the full 256x256x2048 workload after 1 warm-up step, "
+                     f"FP64 oracle (oracle/, OpenMP {threads} threads), {dt:.1f} s"}
+    return rec, A, 1 + n, rho
+
+
+def against_oracle(inp, A_ref, nsteps, rho_ref):
+    """the GPU's FP64 and FP32 fields after the oracle's steps, from the same
+    calibrated input: field rel L2 and the observables' relative errors"""
+    import oracle_lib as O
+    from mpc_amd import Propagator
+    ob = O.observables(inp, A_ref)
+    out = {"steps": nsteps}
+    for prec in ("f64", "f32"):
+        with Propagator(inp, prec) as p:
+            p.upload(inp.calibrated())
+            p.run(0, nsteps)
+            A = p.download()
+            d = p.diagnostics()
+        out[prec] = {"field_rel_l2": O.rel_l2(A, A_ref),
+                     "energy_rel_err": abs(d["energy_sum"] - ob["energy"]) / ob["energy"],
+                     "rms_width_rel_err": abs(d["spec_rms_width"] - ob["width"]) / ob["width"],
+                     "rho_peak_rel_err": abs(d["rho_peak"] - rho_ref) / rho_ref if rho_ref > 0 else None}
+        log(f"[bench] {prec} vs oracle after {nsteps} steps: {out[prec]}")
+        del A
+    return out
 
 
 def batch_bench(steps, warmup):
@@ -230,6 +267,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-pulse batched leg of the default run")
     ap.add_argument("--sweep-variants", action="store_true", help="also time every kernel variant")
     ap.add_argument("--batch", action="store_true",
                     help="instead: the 48-pulse batched workload (FP32), batched vs one by one")
@@ -315,9 +353,24 @@ def main(argv=None):
         valu["frac"] = round(valu["issue_floor_ms"] / kms32[dom], 4)
         roof["valu"] = valu
 
-    cpu = None
+    cpu = vs_oracle = None
     if not a.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        cpu = cpu_baseline(inp)
+        cpu, A_ref, n_ref, rho_ref = cpu_baseline(inp)
+        if a.prec == "both":
+            vs_oracle = against_oracle(inp, A_ref, n_ref, rho_ref)
+        del A_ref
+    batched = None
+    if not a.no_batch and a.prec == "both" and (wl.nx, wl.nt) == (256, 2048):
+        # the 48-pulse workload, batched and one by one (three calls of
+        # bsteps steps stay inside the first 300-step pass)
+        bsteps = max(1, min(a.steps, 60))
+        br = batch_bench(bsteps, bsteps)
+        batched = {"equivalent_steps_per_s": br["value"],
+                   "one_by_one_equivalent_steps_per_s": br["config"]["one_by_one_steps_per_s"],
+                   "batched_over_one_by_one": round(br["value"] / br["config"]["one_by_one_steps_per_s"], 4),
+                   "steps": bsteps, "pulses": 48, "points_total": br["config"]["points_total"],
+                   "workload": br["config"]["workload"], "unit": "256x256x2048-equivalent steps/s",
+                   "dtype": "f32"}
 
     out = {
         "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1,
@@ -329,8 +382,13 @@ def main(argv=None):
                    "precision": ("fp64 complex" if dtype == "f64" else
                                  "fp32 complex, fp64 propagator phase / ADK exponent / density scan"),
                    "plain_fp64_steps_per_s": round(sps64, 3) if sps64 else None,
+                   "passes_per_s": {"f32": round(sps32 / wl.steps_per_pass, 4),
+                                    "f64": round(sps64 / wl.steps_per_pass, 4) if sps64 else None,
+                                    "steps_per_pass": wl.steps_per_pass},
                    "fp32_over_fp64": round(sps32 / sps64, 3) if a.prec == "both" else None,
                    "fp32_vs_fp64_rel_l2": rel,
+                   "vs_oracle": vs_oracle,
+                   "batched": batched,
                    "fp64_kernel_ms": {KERNELS[i]: round(kms64[i], 4) for i in range(3)} if kms64 else None,
                    "autotuned_variants": {"f32": var32, "f64": var64}},
         "roofline": roof,

@@ -853,6 +853,7 @@ extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
 template <class T> static int linear_once(mpc_ctx *c, double length, int sign, bool mirror) {
     const cx<T> *P = get_table<T>(c, 't', length);
     if (!P) return MPC_ENOMEM;
+    c->rho_nvalid = 0;     /* the field no longer is the one the plasma maxima came from */
     RCHK(sweep_t<T>(c, P, nullptr, false, false, true, length));
     return run_xy<T>(c, sign * length, mirror);
 }
@@ -900,6 +901,7 @@ extern "C" int mpc_upload(mpc_ctx *c, const cufftDoubleComplex *A) {
         ? launch_to_internal<float>(c->staging, (cx<float> *)c->field, c->nx, c->ny, c->nt, inv, c->stream)
         : launch_to_internal<double>(c->staging, (cx<double> *)c->field, c->nx, c->ny, c->nt, inv, c->stream);
     c->domain_freq = 0;
+    c->rho_nvalid = 0;     /* no nonlinear step has seen this field yet */
     return finish(c, rc);
 }
 
@@ -931,6 +933,7 @@ extern "C" int mpc_import_planar(mpc_ctx *c, const float *planes, double scale) 
         ? launch_planar<float>((cx<float> *)c->field, (float *)planes, c->N, s, 0, c->stream)
         : launch_planar<double>((cx<double> *)c->field, (float *)planes, c->N, s, 0, c->stream);
     c->domain_freq = 0;
+    c->rho_nvalid = 0;
     return finish(c, rc);
 }
 
@@ -985,7 +988,8 @@ extern "C" int mpc_diagnostics(mpc_ctx *c, mpc_diag *d) {
     d->rho_peak = 0.0;
     if (c->p.plasm == 1) {
         rb.resize((size_t)std::min<long>(c->rho_nvalid, c->rho_nblocks));
-        HIPCHK(hipMemcpy(rb.data(), c->rho_block, sizeof(double) * rb.size(), hipMemcpyDeviceToHost));
+        if (!rb.empty())
+            HIPCHK(hipMemcpy(rb.data(), c->rho_block, sizeof(double) * rb.size(), hipMemcpyDeviceToHost));
         double mx = 0.0;
         for (double v : rb) mx = std::max(mx, v);
         d->rho_peak = mx * c->p.rho_nt;

@@ -130,6 +130,9 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
     __shared__ double lp[LPB][TPL / 64 + 1];
     const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
     for (int i = threadIdx.x; i < NT; i += blockDim.x) { accT[i] = 0.0; accS[i] = 0.0; }
+    /* the zeroing threads are not the ones that add each slot below, and a
+     * block with no line group reaches no barrier inside the loop */
+    __syncthreads();
     double at[E], as[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) { at[m] = 0.0; as[m] = 0.0; }

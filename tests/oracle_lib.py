@@ -139,3 +139,15 @@ def run_fixed(inp, A: np.ndarray, step0: int, nsteps: int, quirks: int = OQ_PHYS
 
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def observables(inp, A: np.ndarray) -> dict:
+    """energy (sum of temp), rms spectral width over the Omega grid, temp and
+    spec of a reference-layout field, from the oracle's temp_spec"""
+    nt = A.shape[0]
+    temp, spec = np.zeros(nt), np.zeros(nt)
+    oracle().oracle_temp_spec(vp(A), ip(dims_arr(A.shape)), dp(temp), dp(spec))
+    om = np.fft.ifftshift(inp.w)                 # the FFT-order Omega of each spectral bin
+    s0, s1, s2 = spec.sum(), (om * spec).sum(), (om * om * spec).sum()
+    width = float(np.sqrt(max(0.0, s2 / s0 - (s1 / s0) ** 2)))
+    return {"energy": float(temp.sum()), "width": width, "temp": temp, "spec": spec}

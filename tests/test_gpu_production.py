@@ -1,0 +1,112 @@
+"""GPU parity at the production sizes, against the FP64 CPU oracle.
+
+* The 256x256x2048 argon workload (1 mJ, 300 fs, 2 bar, 500 steps per pass):
+  steps 498-500, across the end of the first pass (curved mirror after step
+  499), in both precisions: field rel L2 (FP64 <= 1e-10, FP32 <= 1e-4),
+  pulse energy, rms spectral width and peak electron density.
+* The 48-pulse batched workload's three grid classes (128^2 x 512,
+  192^2 x 1024, 256^2 x 2048): one pulse of each, steps 298-300 across the
+  mirror after step 299, run as one batch; each pulse equals its own
+  per-pulse run bit for bit and matches the oracle.
+
+The oracle runs on the host's cores (OpenMP); its results are computed once
+per input and shared by the tests below.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mpc_amd import Batch, Propagator, configs, make_inputs
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+
+def _oracle(key, inp, A0, step0, nsteps):
+    if key not in _cache:
+        ref, rho = O.run_fixed(inp, A0, step0, nsteps)
+        _cache[key] = (ref, rho, O.observables(inp, ref))
+    return _cache[key]
+
+
+def _check(prec, got, d, ref, rho_ref, ob, label):
+    tol = 1e-10 if prec == "f64" else 1e-4
+    err = O.rel_l2(got, ref)
+    e_err = abs(d["energy_sum"] - ob["energy"]) / ob["energy"]
+    w_err = abs(d["spec_rms_width"] - ob["width"]) / ob["width"]
+    r_err = abs(d["rho_peak"] - rho_ref) / rho_ref
+    print(f"  {label} {prec}: field rel_l2={err:.3e} energy={e_err:.3e} width={w_err:.3e} "
+          f"rho_peak={d['rho_peak']:.6e}/{rho_ref:.6e} ({r_err:.3e})", flush=True)
+    assert rho_ref > 0
+    assert err <= tol
+    assert e_err <= tol
+    assert w_err <= tol
+    assert r_err <= (1e-9 if prec == "f64" else 1e-3)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_production_256_across_the_first_mirror(gpu_lib, prec):
+    wl = configs.cfg2_production_256()
+    print(f"case 256x256x2048 steps 498-500 prec={prec}", flush=True)
+    inp = make_inputs(wl)
+    A0 = inp.calibrated()
+    ref, rho_ref, ob = _oracle("cfg2", inp, A0, 498, 3)
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.autotune()
+        p.upload(A0)
+        p.run(498, 3)
+        got = p.download()
+        d = p.diagnostics()
+    _check(prec, got, d, ref, rho_ref, ob, "256x256x2048")
+
+
+def _class_pulses():
+    """the first pulse of each grid class of the 48-pulse workload"""
+    seen, out = set(), []
+    for wl in configs.cfg4_batch():
+        if (wl.nx, wl.nt) not in seen:
+            seen.add((wl.nx, wl.nt))
+            out.append(wl)
+    assert sorted(seen) == [(128, 512), (192, 1024), (256, 2048)]
+    return sorted(out, key=lambda w: w.nt)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batch_grid_classes_bitwise_and_against_oracle(gpu_lib, prec):
+    wls = _class_pulses()
+    cases = []
+    for wl in wls:
+        inp = make_inputs(wl)
+        A0 = inp.calibrated()
+        cases.append((wl, inp, A0))
+    step0, n = 298, 3
+    print(f"case batch classes {[(w.nx, w.nt) for w in wls]} steps {step0}-{step0 + n - 1} prec={prec}",
+          flush=True)
+    # per-pulse runs, each with its class's autotuned variants
+    variants, ref = [], []
+    for wl, inp, A0 in cases:
+        with Propagator(inp, prec) as p:
+            p.upload(A0)
+            variants.append(p.autotune())
+            p.upload(A0)
+            p.run(step0, n)
+            ref.append(p.download())
+    for graph in (False, True):
+        props = []
+        for (wl, inp, A0), v in zip(cases, variants):
+            p = Propagator(inp, prec)
+            p.upload(A0)
+            p.set_variants(*v)
+            props.append(p)
+        with Batch(props) as b:
+            b.run(step0, n, graph=graph)
+            for i, p in enumerate(props):
+                assert np.array_equal(p.download(), ref[i]), (prec, graph, i, variants[i])
+            diags = [p.diagnostics() for p in props]
+        for p in props:
+            p.close()
+    for (wl, inp, A0), got, d in zip(cases, ref, diags):
+        o_ref, rho_ref, ob = _oracle(("cfg4", wl.nx, wl.nt), inp, A0, step0, n)
+        _check(prec, got, d, o_ref, rho_ref, ob, f"{wl.nx}x{wl.nx}x{wl.nt}")
