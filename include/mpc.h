@@ -112,6 +112,22 @@ int  mpc_import_planar(mpc_ctx *ctx, const float *planes, double scale);
 int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int max_steps,
                       int *accepted, double *errs, double *dzs);
 
+/*
+ * setStep (propagation.h:14, propagation.c:616-677) on device-resident
+ * fields, allocating nothing: no counterpart in the reference; it is for
+ * callers that cannot allocate (code inside stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  C1 and C2 are DEVICE pointers to the
+ * dims[0]*dims[1]*dims[2] complex points of the two fields (any point order,
+ * the same in both); the other arguments are host pointers with setStep's
+ * meaning.  The partial sums live in a fixed 64 KB device array the library
+ * holds per GPU until the process ends, which bounds a field to 2^28 points
+ * (MPC_EINVAL beyond).  Runs on the null stream of the current device and
+ * returns once *err is known; nothing is printed.  Calls that overlap on
+ * one GPU share that array and race.
+ */
+int  mpc_setstep_noalloc(const int *dims, double *dzz, const cufftDoubleComplex *C1, const cufftDoubleComplex *C2,
+                         double def_err, double dzmin, const double *y1, const double *y2, int *bad, double *err);
+
 /* FFT/tile autotuner: times every kernel variant (FFT radix plan, tile
  * width, elements per thread) on a scratch copy of the resident field and
  * keeps the fastest in the context (call after mpc_upload).  The variants

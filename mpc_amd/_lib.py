@@ -89,6 +89,7 @@ def lib() -> C.CDLL:
         "mpc_linear": (C.c_int, [vp, C.c_double, C.c_int, C.c_int]),
         "mpc_diagnostics": (C.c_int, [vp, C.POINTER(MpcDiag)]),
         "mpc_run_adaptive": (C.c_int, [vp, dptr, dptr, iptr, C.c_int, iptr, dptr, dptr]),
+        "mpc_setstep_noalloc": (C.c_int, [iptr, dptr, vp, vp, C.c_double, C.c_double, dptr, dptr, iptr, dptr]),
         "mpc_set_timing": (C.c_int, [vp, C.c_int]),
         "mpc_kernel_ms": (C.c_double, [vp, C.c_int]),
         "mpc_field_bytes": (C.c_longlong, [vp]),

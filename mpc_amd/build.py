@@ -31,7 +31,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wno-unused-result"]
 
-HIP_SOURCES = ["tline.hip", "axis.hip", "kernels.hip", "fno.hip", "api.cpp"]
+HIP_SOURCES = ["tline.hip", "axis.hip", "kernels.hip", "fno.hip", "noalloc.hip", "api.cpp"]
 # per-file flags: the persistent sweep kernels keep machine-level LICM from
 # hoisting constants and twiddle products out of their loops into registers
 EXTRA_FLAGS = {"tline.hip": ["-mllvm", "-disable-machine-licm"],

@@ -1000,8 +1000,8 @@ extern "C" int mpc_diagnostics(mpc_ctx *c, mpc_diag *d) {
 /* ------------------------------------------------------------------------ */
 /* adaptive step doubling (MPC.c:343-383, propagation.c:616-677)             */
 /* ------------------------------------------------------------------------ */
-static void setstep_decide(double *dzz, double err, double def_err, double dzmin, const double *y1,
-                           const double *y2, int *bad) {
+void mpc::setstep_decide(double *dzz, double err, double def_err, double dzmin, const double *y1,
+                         const double *y2, int *bad) {
     *bad = 0;
     const double f = std::pow(2.0, 1.0 / 3.0);
     if (err > 2.0 * def_err) {

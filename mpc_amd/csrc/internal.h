@@ -97,6 +97,12 @@ template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const P
                                     cx<double> *tau_F, hipStream_t s);
 int launch_adk(const double *E, long n, const double *kon, double *W1, double *W2, hipStream_t s);
 
+/* setStep's step-size decision on a given error norm (propagation.c:633-677):
+ * halve and reject above 2 tol, shrink by 2^(1/3) above tol, keep above tol/2,
+ * else grow by 2^(1/3) up to 100x the mean beam radius y1[1], y2[1] */
+void setstep_decide(double *dzz, double err, double def_err, double dzmin, const double *y1, const double *y2,
+                    int *bad);
+
 bool tline_supported(int nt);
 bool axis_supported(int n);
 int axis_tile(int precision);   /* G: t values per axis tile */
