@@ -234,16 +234,87 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
     __syncthreads();
 }
 
-/* per-line plasma LDS: U and X, plus S (sigma) and V (1/s) while the
- * workgroup's share stays within 80 KB; otherwise 1/s stays in registers and
- * sigma is formed again for the result */
-template <class G> struct PlasmaLds {
+/* per-line plasma LDS: U and X (FP64) and the scan's scratch, plus S (sigma)
+ * and V (1/s) in the field precision T while the workgroup's share stays
+ * within 80 KB; otherwise 1/s stays in registers and sigma is formed again
+ * for the result */
+template <class G, class T> struct PlasmaLds {
     static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
-    static constexpr int ARR = padded_elems<G::PAD>(NT);   /* doubles per array */
-    static constexpr int SCR = 2 * NW + TPL;
-    static constexpr bool SV = G::LPB * (4 * ARR + SCR) * 8 <= 80 * 1024;
-    static constexpr int NARR = SV ? 4 : 2;
-    static constexpr int BYTES = (NARR * ARR + SCR) * 8;
+    static constexpr int ARR = padded_elems<G::PAD>(NT);   /* elements per array */
+    static constexpr int SCR = 2 * NW + TPL;                /* doubles */
+    static constexpr int SVB = 2 * ARR * (int)sizeof(T);
+    static constexpr bool SV = G::LPB * ((2 * ARR + SCR) * 8 + SVB) <= 80 * 1024;
+    static constexpr int BYTES = (2 * ARR + SCR) * 8 + (SV ? SVB : 0);
+};
+
+/*
+ * The per-element plasma quantities from |A|^2 (a2, the field scale folded
+ * into pk):  1/s (inv_s), the collision time and cross-section (tau, sigma),
+ * the avalanche coefficient a = sigma I dt / (I1 e) and the field-ionisation
+ * coefficient b = W1(1/s) dt/T0 of the first species.
+ *
+ * FP64 path: everything in FP64.  FP32 path: 1/s, tau, sigma and a in FP32;
+ * the ADK exponent and log stay FP64 (an exponent near 100 amplifies any
+ * error of 1/s a hundredfold), formed from |A|^2 in FP64 and 1/s refined by
+ * one FP64 Newton step from the FP32 estimate, and only on lanes whose FP32
+ * exponent says the rate can be non-zero.
+ */
+template <class T> struct PlasmaElem;
+template <> struct PlasmaElem<double> {
+    double inv_s, I;
+    __device__ __forceinline__ void set(double re, double im, const PlasmaK &k) {
+        inv_s = plasma_invs(re * re + im * im, k, &I);
+    }
+    __device__ __forceinline__ double sigma(const PlasmaK &k, double *tau) const { return plasma_sigma(inv_s, k, tau); }
+    __device__ __forceinline__ double avalanche(const PlasmaK &k, double sig) const { return sig * I * k.dt_I1e; }
+    /* ADK rate of species 1 or 2 times Kon */
+    __device__ __forceinline__ double adk_b(const PlasmaK &k, int species) const {
+        return species == 1 ? adk_rate(k.kon1, k.ke1, k.L1, k.kon5, inv_s) * k.Kon
+                            : adk_rate(k.kon2, k.ke2, k.L2, k.kon6, inv_s) * k.Kon;
+    }
+    __device__ __forceinline__ bool adk_live(const PlasmaK &k, int species) const {
+        return species == 1 ? (k.kon1 != 0.0 && k.ke1 * inv_s <= ADK_E_MAX)
+                            : (k.kon2 != 0.0 && k.ke2 * inv_s <= ADK_E_MAX);
+    }
+};
+template <> struct PlasmaElem<float> {
+    float inv_s, I;
+    double a2;                         /* |A|^2 in FP64 (exact from the FP32 components) */
+    __device__ __forceinline__ void set(float re, float im, const PlasmaK &k) {
+        a2 = (double)re * (double)re + (double)im * (double)im;
+        I = (float)a2 * (float)k.hbar_w0;
+        if (I == 0.0f) I = 1e-30f;                           /* plasma.c:135-138 */
+        inv_s = __builtin_amdgcn_rsqf((float)k.zf * I);      /* 1 ulp */
+    }
+    __device__ __forceinline__ float sigma(const PlasmaK &k, float *tau) const {
+        const float tc = (float)k.kt * inv_s;
+        *tau = tc;
+        /* 1 + K5 tc^2 may overflow to inf at the 1e-30 floor: sigma -> 0 there */
+        return (float)k.K4 * tc * __builtin_amdgcn_rcpf(1.0f + (float)k.K5 * (tc * tc));
+    }
+    __device__ __forceinline__ float avalanche(const PlasmaK &k, float sig) const {
+        return sig * I * (float)k.dt_I1e;
+    }
+    /* FP64 1/s: one Newton step from the FP32 estimate (relative error
+     * about 1.5 x (6e-8)^2) */
+    __device__ __forceinline__ double inv_s64(const PlasmaK &k) const {
+        double Id = a2 * k.hbar_w0;
+        if (Id == 0.0) Id = 1e-30;
+        const double x = k.zf * Id, y = (double)inv_s;
+        const double r = fma(-x * y, y, 1.0);
+        return fma(0.5 * y, r, y);
+    }
+    __device__ __forceinline__ bool adk_live(const PlasmaK &k, int species) const {
+        /* the FP32 exponent with a margin for its rounding */
+        return species == 1 ? (k.kon1 != 0.0 && (float)k.ke1 * inv_s <= (float)(ADK_E_MAX * 1.001))
+                            : (k.kon2 != 0.0 && (float)k.ke2 * inv_s <= (float)(ADK_E_MAX * 1.001));
+    }
+    __device__ __forceinline__ double adk_b(const PlasmaK &k, int species) const {
+        if (!adk_live(k, species)) return 0.0;
+        const double iv = inv_s64(k);
+        return species == 1 ? adk_rate(k.kon1, k.ke1, k.L1, k.kon5, iv) * k.Kon
+                            : adk_rate(k.kon2, k.ke2, k.L2, k.kon6, iv) * k.Kon;
+    }
 };
 
 /*
@@ -253,46 +324,52 @@ template <class G> struct PlasmaLds {
  *   ions2_k = (1 - c_k) ions2_{k+1} + c_k u_{k+1},            ions2_NT = 0
  *   ions1_k = u_k - ions2_k ;  F_k = (ions1_k + 2 ions2_k) sigma_k
  * Element k = tid + TPL*m of the line is held by thread tid in slot m.  Two
- * padded LDS arrays: U holds m_k = 1 - a_k, then u_k; X holds the first
+ * padded FP64 LDS arrays: U holds m_k = 1 - a_k, then u_k; X holds the first
  * scan's c_k, then c_k of the second species, then ions2_k.  The second scan
  * forms its coefficients from U and X as it walks, so no per-element state
- * beyond 1/s stays in registers across the scans.
- * sink(m, F, tau) receives the result per element.
+ * beyond 1/s stays in registers across the scans.  The scans and u run in
+ * FP64 in both precisions.
+ * sink(m, F, tau) receives the result per element, in precision T.
  */
 template <class T, class G, class Sink>
 __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, double *lds,
                                             const PlasmaK &pk, double &rho_max, Sink sink) {
-    using L = PlasmaLds<G>;
+    using L = PlasmaLds<G, T>;
     constexpr int NT = G::NT;
     constexpr int E = L::E, TPL = L::TPL;
-    double *U = lds, *X = lds + L::ARR, *scr = lds + L::NARR * L::ARR;
-    double *S = lds + 2 * L::ARR, *V = lds + 3 * L::ARR;   /* used when L::SV */
+    double *U = lds, *X = lds + L::ARR, *scr = lds + 2 * L::ARR;
+    T *S = reinterpret_cast<T *>(lds + 2 * L::ARR + L::SCR), *V = S + L::ARR;   /* used when L::SV */
     int anyc = 0;
-    double invs[L::SV ? 1 : E];
+    T invs[L::SV ? 1 : E];
     __syncthreads();              /* the LDS may still be read by the caller's last exchange */
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = padp_tm<G>(tid, m);                  /* element tid + TPL*m, swizzled */
-        const double a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
-        double I;
-        const double iv = plasma_invs(a2, pk, &I);
-        double tc;
-        const double sig = plasma_sigma(iv, pk, &tc);
-        const double b = adk_rate(pk.kon1, pk.ke1, pk.L1, pk.kon5, iv) * pk.Kon;
-        const double a = sig * I * pk.dt_I1e;             /* avalanche, SI units */
+        PlasmaElem<T> pe;
+        pe.set(A[m].re, A[m].im, pk);
+        T tc;
+        const T sig = pe.sigma(pk, &tc);
+        const double b = pe.adk_b(pk, 1);
+        const double a = (double)pe.avalanche(pk, sig);    /* avalanche, SI units */
         U[k] = 1.0 - a;
         X[k] = b * (1.0 - a) + a;
         if constexpr (L::SV) {
             S[k] = sig;
-            V[k] = iv;
+            V[k] = pe.inv_s;
         } else {
-            invs[m] = iv;
+            invs[m] = pe.inv_s;
         }
-        anyc |= (pk.kon2 != 0.0 && pk.ke2 * iv <= ADK_E_MAX);
+        anyc |= pe.adk_live(pk, 2);
     }
-    auto inv_s = [&](int m) {
-        if constexpr (L::SV) return V[padp_tm<G>(tid, m)];
-        else return invs[m];
+    auto elem = [&](int m) {
+        PlasmaElem<T> pe;
+        if constexpr (std::is_same_v<T, float>) {
+            pe.a2 = (double)A[m].re * (double)A[m].re + (double)A[m].im * (double)A[m].im;
+        } else {
+            pe.I = 0.0;
+        }
+        pe.inv_s = L::SV ? V[padp_tm<G>(tid, m)] : invs[L::SV ? 0 : m];
+        return pe;
     };
     __syncthreads();
     /* U <- u = 1 - exp(-R) */
@@ -302,8 +379,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
     const int block_anyc = __syncthreads_or(anyc);
     if (block_anyc) {
 #pragma unroll
-        for (int m = 0; m < E; ++m)
-            X[padp_tm<G>(tid, m)] = adk_rate(pk.kon2, pk.ke2, pk.L2, pk.kon6, inv_s(m)) * pk.Kon;
+        for (int m = 0; m < E; ++m) X[padp_tm<G>(tid, m)] = elem(m).adk_b(pk, 2);
         __syncthreads();
         /* X <- ions2: m_k = 1 - c_k, c'_k = c_k u_{k+1} */
         affine_suffix_scan<G>(tid, scr,
@@ -321,15 +397,15 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         const double i2 = block_anyc ? X[k] : 0.0;
         const double i1 = U[k] - i2;
         const double fr = i1 + 2.0 * i2;
-        double tc, sig;
+        T tc, sig;
         if constexpr (L::SV) {
             sig = S[k];
-            tc = pk.kt * V[k];
+            tc = (T)pk.kt * V[k];
         } else {
-            sig = plasma_sigma(invs[m], pk, &tc);
+            sig = elem(m).sigma(pk, &tc);
         }
         rho_max = fmax(rho_max, fr);
-        sink(m, fr * sig, tc);
+        sink(m, (T)fr * sig, tc);
     }
     __syncthreads();
 }
@@ -343,7 +419,7 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
 template <class T, class G, bool PLASMA> struct TCfg {
     static constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB, THREADS = G::THREADS;
     static constexpr int FFT1 = G::FFT_ELEMS * (int)sizeof(cx<T>);
-    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<G>::BYTES : 0;
+    static constexpr int PL_BYTES = PLASMA ? PlasmaLds<G, T>::BYTES : 0;
     static constexpr bool DB = LPB * 2 * FFT1 <= 80 * 1024;
     static constexpr int BOFF = DB ? G::FFT_ELEMS : 0;
     static constexpr int FFT_BYTES = (DB ? 2 : 1) * FFT1;

@@ -272,8 +272,8 @@ __global__ void __launch_bounds__((TCfg<T, TGeo<NT>, true>::THREADS)) k_ioniz(co
     for (int m = 0; m < E; ++m) v[m] = f[line * NT + tid + TPL * m];
     double rmax = 0.0;
     plasma_line<T, G>(v, tid, reinterpret_cast<double *>(smem + lslot * C::LINE_BYTES), pk, rmax,
-                       [&](int m, double Fv, double tv) {
-                           out[line * NT + tid + TPL * m] = mk(tv, Fv);   /* (tau_c, F) */
+                       [&](int m, T Fv, T tv) {
+                           out[line * NT + tid + TPL * m] = mk((double)tv, (double)Fv);   /* (tau_c, F) */
                        });
 }
 template <class T, int NT> static int ioniz_go(const cx<T> *f, long nlines, const PlasmaK &pk, cx<double> *out, hipStream_t s) {

@@ -73,13 +73,13 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
             }
         }
         if constexpr (PLASMA) {
-            const double half_rho = 0.5 * a.pk.rho_nt;
+            const T half_rho = (T)(0.5 * a.pk.rho_nt), w0 = (T)a.pk.w0;
             plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
-                               [&](int m, double F, double tau) {
+                               [&](int m, T F, T tau) {
                                    /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
-                                   const double z = half_rho * F;
-                                   d[m].re -= (T)z;
-                                   d[m].im -= (T)(a.pk.w0 * tau * z);
+                                   const T z = half_rho * F;
+                                   d[m].re -= z;
+                                   d[m].im -= w0 * tau * z;
                                });
         }
         /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
