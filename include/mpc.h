@@ -25,6 +25,8 @@ extern "C" {
 #define MPC_ENODEV    (-4)
 #define MPC_EUNSUPP   (-5)
 #define MPC_EIO       (-6)
+#define MPC_ENONFINITE (-7)  /* the guard found a NaN or Inf in the field */
+#define MPC_EGUARD    (-8)   /* the guard's energy or spectral-width check failed */
 
 #define MPC_F64 0   /* plain FP64 path */
 #define MPC_F32 1   /* FP32 complex with FP64 islands */
@@ -77,6 +79,21 @@ void mpc_destroy(mpc_ctx *ctx);
 /* field transfer, reference layout A[t][y][x] (x fastest) */
 int  mpc_upload(mpc_ctx *ctx, const cufftDoubleComplex *A);
 int  mpc_download(mpc_ctx *ctx, cufftDoubleComplex *A);
+/*
+ * Asynchronous checkpoint snapshot (the reference downloads and writes its
+ * checkpoint synchronously, MPC.c:521-544).  mpc_snapshot_begin queues the
+ * resident (time-domain) field's conversion to the reference layout and its
+ * copy into a pinned host buffer the context owns, then returns at once;
+ * runs started afterwards overlap the copy and do not change the snapshot.
+ * mpc_snapshot_poll returns 1 once the copy is complete (or none is
+ * pending), 0 while it runs.  mpc_snapshot_wait blocks until it is complete
+ * and gives the pinned buffer (N points, A[t][y][x]); it stays valid and
+ * unchanged until the next mpc_snapshot_begin or mpc_destroy, so the caller
+ * must be done reading it before it begins another snapshot.
+ */
+int  mpc_snapshot_begin(mpc_ctx *ctx);
+int  mpc_snapshot_poll(mpc_ctx *ctx);
+int  mpc_snapshot_wait(mpc_ctx *ctx, const cufftDoubleComplex **A);
 /* scale the resident field by sqrt(puls_e/Esig) (reference MPC.c:206-216,
  * both parts: the reference scales Re only) */
 int  mpc_calibrate(mpc_ctx *ctx);
@@ -127,6 +144,29 @@ int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int ma
  */
 int  mpc_setstep_noalloc(const int *dims, double *dzz, const cufftDoubleComplex *C1, const cufftDoubleComplex *C2,
                          double def_err, double dzmin, const double *y1, const double *y2, int *bad, double *err);
+
+/*
+ * Solver guard (no counterpart in the reference, whose only guards are step
+ * rejection and NaN -> 0 in the ionisation rates, MPC.c:370-373,
+ * plasma.c:184-192).  mpc_guard_verdict judges a stretch of propagation from
+ * the energies and rms spectral widths before and after it:
+ *   MPC_ENONFINITE  e_after is NaN or infinite (a single non-finite field
+ *                   value makes the energy so), or w_after is when checked;
+ *   MPC_EGUARD      the energy grew by more than gain_tol (relative), or
+ *                   moved by more than energy_tol either way, or the width
+ *                   by more than width_tol;
+ *   0               otherwise.  A tolerance < 0 skips its check.
+ * mpc_set_guard(ctx, gain_tol) makes mpc_run apply it to the resident
+ * field's energy (one extra read of the field before and after each call;
+ * gain_tol < 0, the default, turns it off).  mpc_run then returns the
+ * verdict's code and leaves the field as the run left it.  The pass
+ * surrogate's fallback uses the same verdict with energy_tol and width_tol.
+ */
+int  mpc_guard_verdict(double e_before, double e_after, double w_before, double w_after, double gain_tol,
+                       double energy_tol, double width_tol);
+int  mpc_set_guard(mpc_ctx *ctx, double gain_tol);
+/* sum |A|^2 of the resident field (time or t-spectral domain as it is) */
+int  mpc_energy(mpc_ctx *ctx, double *energy_sum);
 
 /* FFT/tile autotuner: times every kernel variant (FFT radix plan, tile
  * width, elements per thread) on a scratch copy of the resident field and

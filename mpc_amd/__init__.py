@@ -136,6 +136,33 @@ class Propagator:
         check(lib().mpc_download(self.ctx, A.ctypes.data_as(C.c_void_p)), "mpc_download")
         return A
 
+    def set_guard(self, gain_tol: float) -> None:
+        """mpc_run checks the energy stays finite and grows by at most gain_tol (< 0: off)"""
+        check(lib().mpc_set_guard(self.ctx, gain_tol), "mpc_set_guard")
+
+    def energy(self) -> float:
+        e = C.c_double()
+        check(lib().mpc_energy(self.ctx, C.byref(e)), "mpc_energy")
+        return e.value
+
+    def snapshot_begin(self) -> None:
+        """queue an asynchronous copy of the field to the context's pinned host buffer"""
+        check(lib().mpc_snapshot_begin(self.ctx), "mpc_snapshot_begin")
+
+    def snapshot_ready(self) -> bool:
+        rc = lib().mpc_snapshot_poll(self.ctx)
+        if rc < 0:
+            check(rc, "mpc_snapshot_poll")
+        return rc == 1
+
+    def snapshot_wait(self) -> np.ndarray:
+        """the snapshot (a copy of the pinned buffer), reference layout"""
+        p = C.c_void_p()
+        check(lib().mpc_snapshot_wait(self.ctx, C.byref(p)), "mpc_snapshot_wait")
+        n = self.nt * self.ny * self.nx
+        buf = (C.c_double * (2 * n)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.complex128).reshape(self.nt, self.ny, self.nx).copy()
+
     def calibrate(self) -> None:
         check(lib().mpc_calibrate(self.ctx), "mpc_calibrate")
 

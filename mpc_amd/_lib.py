@@ -82,6 +82,12 @@ def lib() -> C.CDLL:
         "mpc_destroy": (None, [vp]),
         "mpc_upload": (C.c_int, [vp, vp]),
         "mpc_download": (C.c_int, [vp, vp]),
+        "mpc_snapshot_begin": (C.c_int, [vp]),
+        "mpc_guard_verdict": (C.c_int, [C.c_double] * 7),
+        "mpc_set_guard": (C.c_int, [vp, C.c_double]),
+        "mpc_energy": (C.c_int, [vp, dptr]),
+        "mpc_snapshot_poll": (C.c_int, [vp]),
+        "mpc_snapshot_wait": (C.c_int, [vp, C.POINTER(C.c_void_p)]),
         "mpc_calibrate": (C.c_int, [vp]),
         "mpc_run": (C.c_int, [vp, C.c_int, C.c_int]),
         "mpc_step": (C.c_int, [vp, C.c_double, C.c_double, iptr]),

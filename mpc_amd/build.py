@@ -66,7 +66,7 @@ def build_library(verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-fopenmp", "-lm"])
     os.replace(tmp, LIB)
     # the run driver (MPC.c's replacement), linked against the library
-    _run(["gcc", *C_FLAGS, "-o", os.path.join(HERE, "mpc"), os.path.join(CSRC, "mpc_main.c"),
+    _run(["gcc", *C_FLAGS, "-pthread", "-o", os.path.join(HERE, "mpc"), os.path.join(CSRC, "mpc_main.c"),
           f"-L{HERE}", "-lmpc_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
     if verbose:
         print("built", LIB, "and", os.path.join(HERE, "mpc"))

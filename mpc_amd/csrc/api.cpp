@@ -60,6 +60,8 @@ extern "C" const char *mpc_strerror(int code) {
     case MPC_ENODEV: return "no GPU device";
     case MPC_EUNSUPP: return "unsupported size or option";
     case MPC_EIO: return "I/O error";
+    case MPC_ENONFINITE: return "non-finite field (guard)";
+    case MPC_EGUARD: return "energy or spectral-width guard failed";
     default: return "unknown error";
     }
 }
@@ -215,6 +217,13 @@ extern "C" int mpc_create(const mpc_params *p, int precision, mpc_ctx **out) {
 extern "C" void mpc_destroy(mpc_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+    if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     free_tables(c);
     void *bufs[] = {c->field, c->aux1, c->aux2, c->staging, c->tw_t, c->tw_x, c->wfft, c->mirror,
                     c->rho_block, c->diag_part, c->diag_line, c->diag_out, c->err_part};
@@ -785,11 +794,46 @@ static int finish(mpc_ctx *c, int rc) {
     return 0;
 }
 
+/* sum |A|^2 of the resident field, in the field's own scale */
+static int field_energy(mpc_ctx *c, double *e) {
+    const int nb = 1024;
+    RCHK(c->prec == MPC_F32 ? launch_energy_partials<float>((const cx<float> *)c->field, c->N, c->err_part, nb, c->stream)
+                            : launch_energy_partials<double>((const cx<double> *)c->field, c->N, c->err_part, nb, c->stream));
+    double part[1024];
+    HIPCHK(hipMemcpyAsync(part, c->err_part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += part[i];
+    *e = s;
+    return 0;
+}
+
+extern "C" int mpc_energy(mpc_ctx *c, double *energy_sum) {
+    if (!c || !energy_sum) return MPC_EINVAL;
+    double e = 0.0;
+    RCHK(field_energy(c, &e));
+    /* the t-spectral field holds F_t(A), whose energy is Nt times larger */
+    *energy_sum = e * c->ascale * c->ascale / (c->domain_freq ? (double)c->nt : 1.0);
+    return 0;
+}
+
+extern "C" int mpc_set_guard(mpc_ctx *c, double gain_tol) {
+    if (!c) return MPC_EINVAL;
+    c->guard_tol = gain_tol;
+    return 0;
+}
+
 extern "C" int mpc_run(mpc_ctx *c, int step0, int nsteps) {
     if (!c || nsteps < 0) return MPC_EINVAL;
     if (nsteps == 0) return 0;
+    const bool guard = c->guard_tol >= 0.0;
+    double e0 = 0.0, e1 = 0.0;
+    if (guard) RCHK(mpc_energy(c, &e0));
     const int rc = c->prec == MPC_F32 ? run_fixed<float>(c, step0, nsteps) : run_fixed<double>(c, step0, nsteps);
-    return finish(c, rc);
+    RCHK(finish(c, rc));
+    if (!guard) return 0;
+    RCHK(mpc_energy(c, &e1));
+    return mpc_guard_verdict(e0, e1, 0.0, 0.0, c->guard_tol, -1.0, -1.0);
 }
 
 /* one isolated split step (propagation.c:111-372 semantics) */
@@ -884,8 +928,16 @@ static double pow2_above(double v) {
     return std::ldexp(1.0, e);
 }
 
+/* work on c->stream that writes the staging buffer waits for a snapshot's
+ * copy out of it */
+static int staging_guard(mpc_ctx *c) {
+    if (c->snap_pending) HIPCHK(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+    return 0;
+}
+
 extern "C" int mpc_upload(mpc_ctx *c, const cufftDoubleComplex *A) {
     if (!c || !A) return MPC_EINVAL;
+    RCHK(staging_guard(c));
     HIPCHK(hipMemcpyAsync(c->staging, A, 16 * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
     /* pick the power-of-two amplitude scale of the resident field */
     const int nb = 1024;
@@ -908,12 +960,63 @@ extern "C" int mpc_upload(mpc_ctx *c, const cufftDoubleComplex *A) {
 extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
     if (!c || !A) return MPC_EINVAL;
     if (c->domain_freq) return MPC_EINVAL;
+    RCHK(staging_guard(c));
     const int rc = c->prec == MPC_F32
         ? launch_to_reference<float>((const cx<float> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream)
         : launch_to_reference<double>((const cx<double> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream);
     RCHK(rc);
     HIPCHK(hipMemcpyAsync(A, c->staging, 16 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
     return finish(c, 0);
+}
+
+/* Checkpoint snapshot (the reference writes resdata.bin synchronously,
+ * MPC.c:521-544): the field in the reference layout goes to the staging
+ * buffer on the context's stream, then to a pinned host buffer on a copy
+ * stream, while the caller goes on stepping. */
+extern "C" int mpc_snapshot_begin(mpc_ctx *c) {
+    if (!c) return MPC_EINVAL;
+    if (c->domain_freq) return MPC_EINVAL;
+    if (!c->copy_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+    }
+    if (!c->pinned && hipHostMalloc((void **)&c->pinned, 16 * (size_t)c->N, hipHostMallocDefault) != hipSuccess) {
+        c->pinned = nullptr;
+        return MPC_ENOMEM;
+    }
+    /* the previous snapshot's copy must be out of the pinned buffer and of staging */
+    if (c->snap_pending) HIPCHK(hipEventSynchronize(c->snap_done));
+    c->snap_pending = 0;
+    const int rc = c->prec == MPC_F32
+        ? launch_to_reference<float>((const cx<float> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream)
+        : launch_to_reference<double>((const cx<double> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream);
+    RCHK(rc);
+    HIPCHK(hipEventRecord(c->snap_ready, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+    HIPCHK(hipMemcpyAsync(c->pinned, c->staging, 16 * (size_t)c->N, hipMemcpyDeviceToHost, c->copy_stream));
+    HIPCHK(hipEventRecord(c->snap_done, c->copy_stream));
+    c->snap_pending = 1;
+    return 0;
+}
+
+extern "C" int mpc_snapshot_poll(mpc_ctx *c) {
+    if (!c) return MPC_EINVAL;
+    if (!c->snap_pending) return 1;
+    const hipError_t e = hipEventQuery(c->snap_done);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    fprintf(stderr, "mpc: HIP error %s in mpc_snapshot_poll\n", hipGetErrorString(e));
+    return MPC_EHIP;
+}
+
+extern "C" int mpc_snapshot_wait(mpc_ctx *c, const cufftDoubleComplex **A) {
+    if (!c || !A) return MPC_EINVAL;
+    if (!c->pinned || !c->snap_done) return MPC_EINVAL;     /* no snapshot was taken */
+    if (c->snap_pending) HIPCHK(hipEventSynchronize(c->snap_done));
+    c->snap_pending = 0;
+    *A = c->pinned;
+    return 0;
 }
 
 /* the resident (time-domain) field as two float planes on the device, in the

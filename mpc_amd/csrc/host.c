@@ -369,3 +369,16 @@ int mpc_adk_constants(const char *gas, double *kon) {
     kon[5] = 2.0 * nq1 - 0.0 - 1.5;
     return MPC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* solver guard verdict (mpc.h)                                              */
+/* ------------------------------------------------------------------------ */
+int mpc_guard_verdict(double e_before, double e_after, double w_before, double w_after, double gain_tol,
+                      double energy_tol, double width_tol) {
+    if (!isfinite(e_after)) return MPC_ENONFINITE;
+    if (width_tol >= 0.0 && !isfinite(w_after)) return MPC_ENONFINITE;
+    if (gain_tol >= 0.0 && e_after > e_before * (1.0 + gain_tol)) return MPC_EGUARD;
+    if (energy_tol >= 0.0 && !(fabs(e_after / e_before - 1.0) <= energy_tol)) return MPC_EGUARD;
+    if (width_tol >= 0.0 && !(fabs(w_after / w_before - 1.0) <= width_tol)) return MPC_EGUARD;
+    return 0;
+}

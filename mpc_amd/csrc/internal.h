@@ -92,6 +92,7 @@ template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt
 int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s);
 template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part,
                                            int nblocks, hipStream_t s);
+template <class T> int launch_energy_partials(const cx<T> *f, long n, double *part, int nblocks, hipStream_t s);
 template <class T> int launch_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n, hipStream_t s);
 template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const PlasmaK &pk,
                                     cx<double> *tau_F, hipStream_t s);
@@ -136,6 +137,13 @@ struct mpc_ctx_s {
     void *field = nullptr;           /* resident field, internal layout */
     void *aux1 = nullptr, *aux2 = nullptr; /* adaptive C1/C2 (lazily) */
     double *staging = nullptr;       /* FP64 reference-layout staging (N complex) */
+    /* checkpoint snapshot: staging -> pinned host copy on copy_stream;
+     * anything that writes staging on `stream` first waits for snap_done */
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    cufftDoubleComplex *pinned = nullptr;
+    int snap_pending = 0;
+    double guard_tol = -1.0;         /* mpc_set_guard: energy-gain tolerance of mpc_run, < 0 off */
     void *tw_t = nullptr, *tw_x = nullptr;   /* twiddles */
     void *wfft = nullptr;            /* FFT-order Omega, precision T */
     void *mirror = nullptr;          /* mirror table [y][x] */

@@ -241,6 +241,28 @@ template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, lon
 template int launch_err_partials<double>(const cx<double> *, const cx<double> *, long, double *, int, hipStream_t);
 template int launch_err_partials<float>(const cx<float> *, const cx<float> *, long, double *, int, hipStream_t);
 
+/* per-block sum |f|^2 in a fixed order (the solver guard's energy) */
+template <class T>
+__global__ void __launch_bounds__(256) k_energy_partials(const cx<T> *f, long n, double *part) {
+    __shared__ double s1[256];
+    double a = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        a += (double)f[i].re * f[i].re + (double)f[i].im * f[i].im;
+    s1[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s1[0];
+}
+template <class T> int launch_energy_partials(const cx<T> *f, long n, double *part, int nb, hipStream_t s) {
+    hipLaunchKernelGGL((k_energy_partials<T>), dim3(nb), dim3(256), 0, s, f, n, part);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template int launch_energy_partials<double>(const cx<double> *, long, double *, int, hipStream_t);
+template int launch_energy_partials<float>(const cx<float> *, long, double *, int, hipStream_t);
+
 template <class T> __global__ void k_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const double r = (4.0 / 3.0) * (double)c2[i].re - (1.0 / 3.0) * (double)c1[i].re;

@@ -23,10 +23,13 @@
  */
 #define _GNU_SOURCE
 #include <getopt.h>
+#include <limits.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include "../../include/mpc.h"
 #include "../../include/utils.h"
@@ -42,6 +45,89 @@ static double *read_doubles(const char *dir, const char *name, long n) {
     if (!v || fread(v, sizeof(double), (size_t)n, f) != (size_t)n) { fprintf(stderr, "mpc: short read %s\n", name); exit(1); }
     fclose(f);
     return v;
+}
+
+/*
+ * Checkpoints (MPC.c:521-544, same files and formats).  The field reaches
+ * the host asynchronously (mpc_snapshot_begin) while the run goes on; once
+ * the copy is complete a writer thread writes resdata.bin, resparam.bin and
+ * the .bak copies of the append files, each .bak cut to the size its file
+ * had at the checkpoint (the run keeps appending meanwhile).  Everything is
+ * written under a temporary name first and renamed at the end, so a run
+ * killed mid-write leaves the previous checkpoint in place.
+ */
+#define NAPPEND 5
+static const char *APPEND_NAMES[NAPPEND] = {"bp1.bin", "bp2.bin", "temp.bin", "spec.bin", "other.bin"};
+
+typedef struct {
+    char dir[4096];
+    const cufftDoubleComplex *A;     /* the pinned snapshot */
+    long N;
+    double dzz, z, puls_e;
+    int signmem, stepn;
+    long sizes[NAPPEND];             /* append files' sizes at the checkpoint */
+    int rc;
+} ckpt_job;
+
+static int write_prefix(const char *from, const char *to, long nbytes) {
+    FILE *a = fopen(from, "rb");
+    if (!a) return -1;
+    FILE *b = fopen(to, "wb");
+    if (!b) { fclose(a); return -1; }
+    char buf[1 << 16];
+    int rc = 0;
+    while (nbytes > 0) {
+        const size_t want = nbytes < (long)sizeof buf ? (size_t)nbytes : sizeof buf;
+        const size_t got = fread(buf, 1, want, a);
+        if (got != want || fwrite(buf, 1, got, b) != got) { rc = -1; break; }
+        nbytes -= (long)got;
+    }
+    fclose(a);
+    if (fclose(b) != 0) rc = -1;
+    return rc;
+}
+
+static void *ckpt_writer(void *arg) {
+    ckpt_job *j = (ckpt_job *)arg;
+    char path[4300], tmp[4400];
+    j->rc = 0;
+    snprintf(tmp, sizeof tmp, "%s/resdata.bin.tmp", j->dir);
+    FILE *f = fopen(tmp, "wb");
+    if (!f || fwrite(j->A, sizeof(cufftDoubleComplex), (size_t)j->N, f) != (size_t)j->N) j->rc = -1;
+    if (f && fclose(f) != 0) j->rc = -1;
+    snprintf(tmp, sizeof tmp, "%s/resparam.bin.tmp", j->dir);
+    f = fopen(tmp, "wb");
+    if (!f) j->rc = -1;
+    else {
+        int ok = fwrite(&j->dzz, 8, 1, f) == 1 && fwrite(&j->z, 8, 1, f) == 1 && fwrite(&j->signmem, 4, 1, f) == 1 &&
+                 fwrite(&j->puls_e, 8, 1, f) == 1 && fwrite(&j->stepn, 4, 1, f) == 1;
+        if (fclose(f) != 0 || !ok) j->rc = -1;
+    }
+    for (int i = 0; i < NAPPEND && !j->rc; i++) {
+        snprintf(path, sizeof path, "%s/%s", j->dir, APPEND_NAMES[i]);
+        snprintf(tmp, sizeof tmp, "%s/%s.bak.tmp", j->dir, APPEND_NAMES[i]);
+        if (write_prefix(path, tmp, j->sizes[i]) != 0) j->rc = -1;
+    }
+    if (!j->rc) {
+        const char *names[2] = {"resdata.bin", "resparam.bin"};
+        for (int i = 0; i < 2 + NAPPEND && !j->rc; i++) {
+            if (i < 2) {
+                snprintf(path, sizeof path, "%s/%s", j->dir, names[i]);
+            } else {
+                snprintf(path, sizeof path, "%s/%s.bak", j->dir, APPEND_NAMES[i - 2]);
+            }
+            snprintf(tmp, sizeof tmp, "%s.tmp", path);
+            if (rename(tmp, path) != 0) j->rc = -1;
+        }
+    }
+    if (j->rc) fprintf(stderr, "mpc: writing the checkpoint in %s failed\n", j->dir);
+    return NULL;
+}
+
+static long file_size(FILE *fp) {
+    struct stat st;
+    fflush(fp);
+    return fstat(fileno(fp), &st) == 0 ? (long)st.st_size : -1;
 }
 
 int main(int argc, char **argv) {
@@ -82,7 +168,7 @@ int main(int argc, char **argv) {
     if ((rc = mpc_create(&p, prec, &ctx))) die("mpc_create", rc);
 
     FILE *fbp1, *fbp2, *ftemp, *fspec, *foth;
-    char resd[4300], resp[4300];
+    char resd[4300], resp[4300];   /* read on resume; written by the checkpoint writer */
     snprintf(resd, sizeof resd, "%s/resdata.bin", cpath);
     snprintf(resp, sizeof resp, "%s/resparam.bin", cpath);
     cufftDoubleComplex *A = (cufftDoubleComplex *)malloc(sizeof(cufftDoubleComplex) * (size_t)N);
@@ -121,6 +207,13 @@ int main(int argc, char **argv) {
     double *BP1 = (double *)malloc(sizeof(double) * p.dims[0]), *BP2 = (double *)malloc(sizeof(double) * p.dims[1]);
     printf("Starting main loop on the GPU!\n");
     int done = 0;
+    /* checkpoint state: 0 none, 1 snapshot copying, 2 writer thread running */
+    int ck_state = 0;
+    ckpt_job job;
+    pthread_t writer;
+    memset(&job, 0, sizeof job);
+    snprintf(job.dir, sizeof job.dir, "%s", cpath);
+    job.N = N;
     while (z < p.dist && (max_steps < 0 || done < max_steps)) {
         if (fixed_S > 0) {
             /* fixed schedule: dump_every steps resident on the device */
@@ -154,23 +247,41 @@ int main(int argc, char **argv) {
         fprintf(foth, "%f\t%f\n", err, dzz); fflush(foth);
         if (z > p.dist) z = p.dist;
         printf("We are here: %5.2f\n", z * 100.0 / p.dist);
+        /* a finished snapshot goes to the writer thread */
+        if (ck_state == 1 && mpc_snapshot_poll(ctx) == 1) {
+            if ((rc = mpc_snapshot_wait(ctx, &job.A))) die("snapshot", rc);
+            if (pthread_create(&writer, NULL, ckpt_writer, &job) != 0) { fprintf(stderr, "mpc: no writer thread\n"); return 1; }
+            ck_state = 2;
+        }
         if ((nstep % 10) == 0) {
             /* checkpoint (MPC.c:521-544) */
             printf("We are at step %d, writing checkpoint data into files.\n", nstep);
-            if ((rc = mpc_download(ctx, A))) die("download", rc);
-            int stepn = nstep + 1;
-            FILE *f = fopen(resd, "wb");
-            if (!f) { fprintf(stderr, "mpc: cannot write %s\n", resd); return 1; }
-            fwrite(A, sizeof(cufftDoubleComplex), (size_t)N, f);
-            fclose(f);
-            f = fopen(resp, "wb");
-            if (!f) { fprintf(stderr, "mpc: cannot write %s\n", resp); return 1; }
-            fwrite(&dzz, 8, 1, f); fwrite(&z, 8, 1, f); fwrite(&signmem, 4, 1, f);
-            fwrite(&puls_e, 8, 1, f); fwrite(&stepn, 4, 1, f);
-            fclose(f);
-            backup(fbp1); backup(fbp2); backup(ftemp); backup(fspec); backup(foth);
+            if (ck_state == 1) {              /* the previous one is still copying: finish it first */
+                if ((rc = mpc_snapshot_wait(ctx, &job.A))) die("snapshot", rc);
+                ckpt_writer(&job);
+                if (job.rc) return 1;
+                ck_state = 0;
+            }
+            if (ck_state == 2) {              /* its writer must be done with the pinned buffer */
+                pthread_join(writer, NULL);
+                if (job.rc) return 1;
+                ck_state = 0;
+            }
+            if ((rc = mpc_snapshot_begin(ctx))) die("snapshot", rc);
+            job.dzz = dzz; job.z = z; job.signmem = signmem; job.puls_e = puls_e; job.stepn = nstep + 1;
+            FILE *fa[NAPPEND] = {fbp1, fbp2, ftemp, fspec, foth};
+            for (int i = 0; i < NAPPEND; i++) job.sizes[i] = file_size(fa[i]);
+            ck_state = 1;
         }
         nstep++;
+    }
+    if (ck_state == 1) {
+        if ((rc = mpc_snapshot_wait(ctx, &job.A))) die("snapshot", rc);
+        ckpt_writer(&job);
+        if (job.rc) return 1;
+    } else if (ck_state == 2) {
+        pthread_join(writer, NULL);
+        if (job.rc) return 1;
     }
     if ((rc = mpc_download(ctx, A))) die("download", rc);
     FILE *fo = opener(cpath, "out.bin", "wb", 0);

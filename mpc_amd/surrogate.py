@@ -19,8 +19,9 @@ network is then that linear map, and the held-out error of each is reported.
 
 Rollout: every plain_every-th pass, and every pass whose prediction fails
 the energy check (|E_pred / E_in - 1| > energy_tol) or the spectral-width
-check (|w_pred / w_in - 1| > width_tol), is run by the plain split step; the
-other passes are the network's.
+check (|w_pred / w_in - 1| > width_tol), or is not finite, is run by the
+plain split step; the other passes are the network's.  The checks are the
+solver guard's verdict (mpc_guard_verdict; a tolerance < 0 skips its check).
 """
 from __future__ import annotations
 
@@ -225,10 +226,11 @@ def rollout(net: FnoGpu, wl, passes: int, plain_every: int = 5, energy_tol: floa
             t_net.append(time.perf_counter() - t1)
             import_(p, pred, 0, scale)
             d_out = p.diagnostics()
-            e_ok = abs(d_out["energy_sum"] / d_in["energy_sum"] - 1.0) <= energy_tol
-            w_ok = abs(d_out["spec_rms_width"] / d_in["spec_rms_width"] - 1.0) <= width_tol
+            # the solver guard's verdict, two-sided on energy and width
+            verdict = lib().mpc_guard_verdict(d_in["energy_sum"], d_out["energy_sum"], d_in["spec_rms_width"],
+                                              d_out["spec_rms_width"], -1.0, energy_tol, width_tol)
             d_cur = d_out
-            if not (e_ok and w_ok):
+            if verdict != 0:
                 import_(p, start, 0, scale)      # back to the pass's input, then the plain pass
                 kind = "fallback"
         if kind == "net":

@@ -129,12 +129,20 @@ template <int N> int run_t() {
     return bad;
 }
 
-int main() {
+/* usage: fft_emul [max_n] (default 4096: every size) */
+int main(int argc, char **argv) {
+    const int maxn = argc > 1 ? atoi(argv[1]) : 4096;
     int bad = 0;
-    bad |= run_t<512>(); bad |= run_t<1024>(); bad |= run_t<2048>(); bad |= run_t<4096>();
+    if (maxn >= 512) bad |= run_t<512>();
+    if (maxn >= 1024) bad |= run_t<1024>();
+    if (maxn >= 2048) bad |= run_t<2048>();
+    if (maxn >= 4096) bad |= run_t<4096>();
     bad |= run_n<16>(); bad |= run_n<32>(); bad |= run_n<64>(); bad |= run_n<128>();
-    bad |= run_n<192>(); bad |= run_n<256>(); bad |= run_n<512>(); bad |= run_n<1024>();
-    bad |= run_n<2048>(); bad |= run_n<4096>();
+    bad |= run_n<192>(); bad |= run_n<256>();
+    if (maxn >= 512) bad |= run_n<512>();
+    if (maxn >= 1024) bad |= run_n<1024>();
+    if (maxn >= 2048) bad |= run_n<2048>();
+    if (maxn >= 4096) bad |= run_n<4096>();
     printf(bad ? "FAIL\n" : "OK\n");
     return bad;
 }

@@ -155,3 +155,28 @@ def test_unknown_gas_and_unsupported_sizes_are_rejected():
     kon = np.zeros(6)
     assert lib().mpc_adk_constants(b"Unobtainium", O.dp(kon)) != 0
     assert lib().mpc_adk_constants(b"Argon", O.dp(kon)) == 0
+
+
+def test_guard_verdict_on_injected_non_finite_values():
+    """the solver guard's verdict (host logic): NaN / Inf energies or widths,
+    energy gain beyond its tolerance, two-sided energy and width checks"""
+    L = lib()
+    nan, inf = float("nan"), float("inf")
+    ENONFINITE, EGUARD = -7, -8
+    v = L.mpc_guard_verdict
+    assert v(1.0, 1.0, 1.0, 1.0, 1e-3, 1e-3, 1e-3) == 0
+    assert v(1.0, nan, 1.0, 1.0, 1e-3, -1, -1) == ENONFINITE
+    assert v(1.0, inf, 1.0, 1.0, -1, -1, -1) == ENONFINITE       # checked even with every tolerance off
+    assert v(1.0, -inf, 1.0, 1.0, -1, -1, -1) == ENONFINITE
+    assert v(1.0, 1.0, 1.0, nan, -1, -1, 0.05) == ENONFINITE
+    assert v(1.0, 1.0, 1.0, nan, -1, -1, -1) == 0                 # width not checked
+    # one-sided gain check (the solver): losses pass, gains beyond the tolerance fail
+    assert v(1.0, 0.5, 0, 0, 1e-2, -1, -1) == 0
+    assert v(1.0, 1.009, 0, 0, 1e-2, -1, -1) == 0
+    assert v(1.0, 1.011, 0, 0, 1e-2, -1, -1) == EGUARD
+    # two-sided checks (the surrogate's fallback)
+    assert v(1.0, 0.97, 1.0, 1.0, -1, 0.02, 0.05) == EGUARD
+    assert v(1.0, 1.01, 1.0, 1.06, -1, 0.02, 0.05) == EGUARD
+    assert v(1.0, 1.01, 1.0, 1.04, -1, 0.02, 0.05) == 0
+    assert v(0.0, 0.0, 1.0, 1.0, -1, 0.02, -1) == EGUARD            # 0/0 is not within any tolerance
+    assert lib().mpc_strerror(ENONFINITE) and lib().mpc_strerror(EGUARD)

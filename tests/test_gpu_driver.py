@@ -67,3 +67,58 @@ def test_driver_adaptive_runs(gpu_lib, tmp_path):
     assert "local error" in out or "We are here" in out
     lines = open(os.path.join(rundir, "other.bin")).read().strip().splitlines()
     assert len(lines) == 4 and all("\t" in l for l in lines)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_snapshot_is_the_field_at_its_begin(gpu_lib, prec):
+    """mpc_snapshot_begin queues the copy and returns; steps run afterwards
+    (across a mirror) and a download in between do not change what the
+    snapshot holds; a second snapshot replaces it"""
+    from mpc_amd import Propagator
+    print(f"case snapshot prec={prec}", flush=True)
+    inp = make_inputs(configs.small(32, 512, steps_per_pass=3, passes=3))
+    A0 = inp.calibrated()
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, 2)
+        at2 = p.download()
+        p.snapshot_begin()
+        p.run(2, 4)
+        at6 = p.download()
+        snap = p.snapshot_wait()
+        assert np.array_equal(snap, at2)
+        assert p.snapshot_ready()
+        p.snapshot_begin()
+        p.run(6, 1)
+        assert np.array_equal(p.snapshot_wait(), at6)
+        assert not np.array_equal(at6, at2)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_guard_stops_a_run_on_a_non_finite_field(gpu_lib, prec):
+    """with the guard on, a field holding a NaN makes mpc_run fail with
+    MPC_ENONFINITE; a normal run passes the guard with the same bits as one
+    without it"""
+    from mpc_amd import Propagator
+    print(f"case guard prec={prec}", flush=True)
+    inp = make_inputs(configs.small(32, 512, steps_per_pass=3, passes=2))
+    A0 = inp.calibrated()
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, 4)
+        ref = p.download()
+        p.upload(A0)
+        p.set_guard(1e-2)
+        e0 = p.energy()
+        assert abs(e0 / (np.abs(A0) ** 2).sum() - 1) < 1e-6
+        p.run(0, 4)
+        assert np.array_equal(p.download(), ref)
+        bad = A0.copy()
+        bad[100, 5, 7] = np.nan
+        p.upload(bad)
+        with pytest.raises(RuntimeError, match="non-finite"):
+            p.run(0, 1)
+        p.set_guard(-1.0)
+        p.upload(bad)
+        p.run(0, 1)                                # unguarded: runs, the NaN spreads
+        assert not np.isfinite(p.download()).all()
