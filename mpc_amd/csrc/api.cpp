@@ -226,7 +226,7 @@ extern "C" void mpc_destroy(mpc_ctx *c) {
     if (c->pinned) (void)hipHostFree(c->pinned);
     free_tables(c);
     void *bufs[] = {c->field, c->aux1, c->aux2, c->staging, c->tw_t, c->tw_x, c->wfft, c->mirror,
-                    c->rho_block, c->diag_part, c->diag_line, c->diag_out, c->err_part};
+                    c->rho_block, c->diag_part, c->diag_line, c->diag_out, c->err_part, c->rich_err, c->rich_lp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : c->evs) {
@@ -859,7 +859,15 @@ template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *s
  * half steps merged (their linear factors commute): x y | t | x y (+P on the
  * t-spectral field) | t | x y, 8 sweeps instead of 10.  signmem is updated
  * as two step_once calls would. */
-template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem, const void *src = nullptr) {
+/* the fused end of an adaptive attempt: C1's buffer and the partial sums
+ * the last y sweep of C2 writes (launch_axis_richardson) */
+struct RichOut {
+    const void *c1 = nullptr;
+    double *err_part = nullptr, *lp_part = nullptr;
+};
+
+template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem, const void *src = nullptr,
+                                        const RichOut *rich = nullptr) {
     int s[2];
     for (int i = 0; i < 2; i++) {
         const double zi = z + i * dzz;
@@ -880,7 +888,18 @@ template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *s
     RCHK(sweep_t<T>(c, P, nullptr, false, true, false, dzz));           /* leaves the field t-spectral */
     RCHK(run_xy<T>(c, (s[0] + s[1]) * dzz / 2, false, P2));            /* x P(dz) on the y sweep */
     RCHK(sweep_t<T>(c, nullptr, P, true, true, true, dzz));
-    return run_xy<T>(c, s[1] * dzz / 2, false);
+    if (!rich) return run_xy<T>(c, s[1] * dzz / 2, false);
+    /* last x sweep as usual, then the y sweep fused with the error norms,
+     * the per-line powers and the Richardson combination */
+    const cx<T> *D = get_table<T>(c, 'x', s[1] * dzz / 2);
+    if (!D) return MPC_ENOMEM;
+    RCHK(sweep_axis<T>(c, 0, D, false));
+    AArgs<T> a = axis_args<T>(c, 1, D, false, nullptr);
+    tic(c, 2);
+    const int rc = launch_axis_richardson<T>(a, c->ny, (const cx<T> *)rich->c1, rich->err_part, rich->lp_part,
+                                             c->stream);
+    toc(c, 2);
+    return rc;
 }
 
 extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
@@ -1120,57 +1139,64 @@ void mpc::setstep_decide(double *dzz, double err, double def_err, double dzmin, 
     }
 }
 
-template <class T> static int err_norms(mpc_ctx *c, const void *c1, const void *c2, double *num, double *den) {
-    const int nb = 1024;
-    RCHK(launch_err_partials<T>((const cx<T> *)c1, (const cx<T> *)c2, c->N, c->err_part, nb, c->stream));
-    std::vector<double> part(2 * nb);
-    HIPCHK(hipMemcpyAsync(part.data(), c->err_part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    double a = 0, b = 0;
-    for (int i = 0; i < nb; i++) { a += part[2 * i]; b += part[2 * i + 1]; }
-    *num = a; *den = b;
-    return 0;
-}
-
+/*
+ * Adaptive step doubling + Richardson (MPC.c:343-383): per attempt C1 = one
+ * 2dz step and C2 = two dz steps, both from A.  C2's last sweep is fused
+ * with the error norms, the per-line powers of C2 (BP1/BP2 -> calc1oes, the
+ * step-size cap) and R = 4/3 C2 - 1/3 C1, which it leaves in C2's buffer;
+ * an accepted step adopts that buffer as the field.  One host round trip per
+ * attempt brings back the two norms and the line powers.
+ */
 template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem, int max_steps,
                                        int *accepted, double *errs, double *dzs) {
     const size_t bytes = c->esz * (size_t)c->N;
     if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
     if (!c->aux2 && hipMalloc(&c->aux2, bytes) != hipSuccess) return MPC_ENOMEM;
-    void *A = c->field;
+    const long ntiles = axis_richardson_tiles(c->prec, c->nt, c->nx);
+    const long nchunks = ntiles / c->nx;
+    if (!c->rich_err && hipMalloc((void **)&c->rich_err, sizeof(double) * (2 * (size_t)ntiles + 2)) != hipSuccess)
+        return MPC_ENOMEM;
+    if (!c->rich_lp && hipMalloc((void **)&c->rich_lp, sizeof(double) * (size_t)nchunks * c->nxy) != hipSuccess)
+        return MPC_ENOMEM;
     int acc = 0, guard = 0;
-    std::vector<double> BP1(c->nx), BP2(c->ny);
-    double y1[4] = {0, 0, 0, 0}, y2[4] = {0, 0, 0, 0}, bps = 0;
+    std::vector<double> BP1(c->nx), BP2(c->ny), lp((size_t)c->nxy);
+    double y1[4] = {0, 0, 0, 0}, y2[4] = {0, 0, 0, 0}, bps = 0, norms[2];
+    RichOut ro;
+    ro.err_part = c->rich_err;
+    ro.lp_part = c->rich_lp;
     while (*z < c->p.dist && acc < max_steps && guard < 64 * max_steps + 64) {
         guard++;
         const double dz = *dzz;
+        void *A = c->field;
         /* C1 and C2 start from A without copies: each one's first sweep reads
          * A and writes its own buffer */
         int sm1 = *signmem;
         c->field = c->aux1;
         int rc = step_once<T>(c, 2 * dz, *z, &sm1, A);
         c->field = c->aux2;
-        if (!rc) rc = two_steps<T>(c, dz, *z, signmem, A);       /* sub-steps at z and z + dz */
-        if (!rc) {
-            /* BP and calc1oes of C2 for the step-size cap (propagation.c:343-362) */
-            mpc_diag dg{};
-            dg.BP1 = BP1.data(); dg.BP2 = BP2.data();
-            rc = mpc_diagnostics(c, &dg);
-            if (!rc) {
-                int dims[3] = {c->nx, c->ny, c->nt};
-                calc1oes(BP1.data(), dims, c->r.data(), &bps, y1);
-                calc1oes(BP2.data(), dims, c->r.data(), &bps, y2);
-            }
-        }
+        ro.c1 = c->aux1;
+        if (!rc) rc = two_steps<T>(c, dz, *z, signmem, A, &ro);   /* sub-steps at z and z + dz */
         c->field = A;
         RCHK(rc);
-        double num = 0, den = 0;
-        RCHK(err_norms<T>(c, c->aux1, c->aux2, &num, &den));
-        const double err = std::sqrt(num) / std::sqrt(den);
+        RCHK(launch_sum_pairs(c->rich_err, ntiles, c->rich_err + 2 * ntiles, c->stream));
+        RCHK(launch_reduce_cols(c->rich_lp, (int)nchunks, (int)c->nxy, c->diag_line, c->stream));
+        HIPCHK(hipMemcpyAsync(norms, c->rich_err + 2 * ntiles, sizeof norms, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(lp.data(), c->diag_line, sizeof(double) * c->nxy, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->timing) resolve_timing(c);
+        /* BP and calc1oes of C2 for the step-size cap (propagation.c:343-362) */
+        for (int i = 0; i < c->nx; i++) { double sm = 0; for (int j = 0; j < c->ny; j++) sm += lp[(size_t)j * c->nx + i]; BP1[i] = sm; }
+        for (int j = 0; j < c->ny; j++) { double sm = 0; for (int i = 0; i < c->nx; i++) sm += lp[(size_t)j * c->nx + i]; BP2[j] = sm; }
+        int dims[3] = {c->nx, c->ny, c->nt};
+        calc1oes(BP1.data(), dims, c->r.data(), &bps, y1);
+        calc1oes(BP2.data(), dims, c->r.data(), &bps, y2);
+        const double err = std::sqrt(norms[0]) / std::sqrt(norms[1]);
         int bad = 0;
         setstep_decide(dzz, err, c->p.def_err, c->p.dzmin, y1, y2, &bad);
         if (bad) continue;
-        RCHK(launch_richardson<T>((cx<T> *)A, (const cx<T> *)c->aux1, (const cx<T> *)c->aux2, c->N, c->stream));
+        /* accept: the Richardson field is in C2's buffer */
+        c->field = c->aux2;
+        c->aux2 = A;
         const double zold = *z;
         *z = *z + 2.0 * dz;
         if (*z > c->p.dist) *z = c->p.dist;

@@ -262,6 +262,123 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
     }
 }
 
+/*
+ * The last y sweep of an adaptive attempt's fine branch C2 (two dz steps),
+ * fused with what the step-size control needs (propagation.c:343-362,
+ * :616-631; MPC.c:377-381).  Per tile (x, t-chunk) it finishes C2 as the y
+ * sweep does, then, with C1 (one 2dz step) read at the same points:
+ *   err_part[2 tile + 0/1] = sum |C2 - C1|^2, sum |C2|^2      (FP64)
+ *   lp_part[chunk][y][x]   = sum over the chunk's t of |C2|^2 (FP64, t order)
+ * and stores R = 4/3 C2 - 1/3 C1 (formed in FP64) in place of C2, so an
+ * accepted step only has to adopt that buffer.  One workgroup per tile,
+ * chunks of each x consecutive; the plan is the 8-per-thread one where its
+ * workgroup fits in 1024 threads, else Plan<N>.
+ */
+template <class T, int N> struct RichCfg {
+    static constexpr int G = ATile<T>::G;
+    using PL = std::conditional_t<(G * (N / APlan8<N>::E) <= 1024), APlan8<N>, Plan<N>>;
+    using C = ACfg<T, N, PL, G>;
+};
+template <class T, int N>
+__global__ void __launch_bounds__((RichCfg<T, N>::C::THREADS))
+k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__ err_part,
+                  double *__restrict__ lp_part, long nxy, int nx) {
+    using RC = RichCfg<T, N>;
+    using PL = typename RC::PL;
+    using C = typename RC::C;
+    constexpr int E = C::E, TPL = C::TPL, G = C::G, THREADS = C::THREADS;
+    __shared__ __align__(16) cx<T> lds[N * G];
+    constexpr int W = THREADS < 64 ? THREADS : 64, NWV = THREADS / W;   /* lanes per wave, waves */
+    __shared__ double wred[2 * NWV];
+    const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
+    const long lin = blockIdx.x;
+    long outer, t0;
+    axis_tile_of(a, lin, outer, t0, G);
+    const long off0 = outer * a.s_outer + t0 + gcol;
+    const cx<T> *in = a.src ? a.src : a.field;
+    cx<T> v[E], Dc[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = gload(in + off0, (long)(tid + TPL * m) * a.s_line);
+    const cx<T> p = a.Pt ? gload(a.Pt, t0 + gcol) : mk((T)1, (T)0);
+    auto fft = [&](cx<T>(&x)[E], auto dir) {
+        constexpr int DIR = decltype(dir)::value;
+        line_fft<T, N, PL, DIR>(x, tid, lds + gcol, G, a.tw);
+    };
+    axis_apply<T, N, PL, G, false>(a, v, p, outer, tid, Dc, fft);
+    __syncthreads();                                  /* the transform's last LDS reads are done */
+    double *pw_t = reinterpret_cast<double *>(lds);   /* [line along y][t in chunk] */
+    double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const long off = (long)(tid + TPL * m) * a.s_line;
+        const cx<T> u = gload(c1 + off0, off);
+        const double cr = (double)v[m].re, ci = (double)v[m].im, ur = (double)u.re, ui = (double)u.im;
+        const double dr = cr - ur, di = ci - ui, pw = cr * cr + ci * ci;
+        e1 += dr * dr + di * di;
+        e2 += pw;
+        pw_t[(tid + TPL * m) * G + gcol] = pw;
+        gstore(a.field + off0, off, mk((T)((4.0 / 3.0) * cr - (1.0 / 3.0) * ur), (T)((4.0 / 3.0) * ci - (1.0 / 3.0) * ui)));
+    }
+    /* block sums of the two error norms: waves, then the wave totals in order */
+    for (int o = W / 2; o > 0; o >>= 1) {
+        e1 += __shfl_down(e1, o, W);
+        e2 += __shfl_down(e2, o, W);
+    }
+    if (threadIdx.x % W == 0) {
+        wred[2 * (threadIdx.x / W)] = e1;
+        wred[2 * (threadIdx.x / W) + 1] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int w = 0; w < NWV; ++w) { s1 += wred[2 * w]; s2 += wred[2 * w + 1]; }
+        err_part[2 * lin] = s1;
+        err_part[2 * lin + 1] = s2;
+    }
+    /* per-line power over the chunk's t values, in t order: line y at x = outer */
+    const long chunk = t0 / G;
+    for (int j = threadIdx.x; j < N; j += THREADS) {
+        double sum = 0.0;
+        for (int g = 0; g < G; ++g) sum += pw_t[j * G + g];
+        lp_part[chunk * nxy + (long)j * nx + outer] = sum;
+    }
+}
+template <class T, int N>
+static int rich_go(AArgs<T> a, int nx, const cx<T> *c1, double *err_part, double *lp_part, hipStream_t s) {
+    using C = typename RichCfg<T, N>::C;
+    if (a.nt % C::G) return MPC_EINVAL;
+    const int nch = a.nt / C::G;
+    a.n_outer = nx;
+    a.tg = nch;                                        /* the chunks of one x consecutive */
+    const long nb = (long)nch * nx;
+    if (nb <= 0 || nb > 0x7fffffffL) return MPC_EINVAL;
+    hipLaunchKernelGGL((k_axis_richardson<T, N>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a, c1, err_part, lp_part,
+                       (long)nx * N, nx);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+template <class T>
+int launch_axis_richardson(const AArgs<T> &a, int n, const cx<T> *c1, double *err_part, double *lp_part, hipStream_t s) {
+    switch (n) {
+    case 16:  return rich_go<T, 16>(a, n, c1, err_part, lp_part, s);
+    case 32:  return rich_go<T, 32>(a, n, c1, err_part, lp_part, s);
+    case 64:  return rich_go<T, 64>(a, n, c1, err_part, lp_part, s);
+    case 128: return rich_go<T, 128>(a, n, c1, err_part, lp_part, s);
+    case 192: return rich_go<T, 192>(a, n, c1, err_part, lp_part, s);
+    case 256: return rich_go<T, 256>(a, n, c1, err_part, lp_part, s);
+    case 512: return rich_go<T, 512>(a, n, c1, err_part, lp_part, s);
+    default: return MPC_EUNSUPP;
+    }
+}
+template int launch_axis_richardson<double>(const AArgs<double> &, int, const cx<double> *, double *, double *,
+                                            hipStream_t);
+template int launch_axis_richardson<float>(const AArgs<float> &, int, const cx<float> *, double *, double *,
+                                           hipStream_t);
+/* tiles of the fused sweep (its partial-sum slots): t-chunks x Nx */
+int axis_richardson_tiles(int precision, int nt, int nx) {
+    const int G = precision == MPC_F32 ? ATile<float>::G : ATile<double>::G;
+    return nt / G * nx;
+}
+
 bool axis_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
 int axis_variants(void) { return 32; }
 

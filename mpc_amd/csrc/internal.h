@@ -92,8 +92,14 @@ template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt
 int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s);
 template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part,
                                            int nblocks, hipStream_t s);
+/* the last y sweep of an adaptive fine branch fused with the error norms,
+ * per-line powers and the Richardson combination (axis.hip); n = Ny = Nx */
+template <class T> int launch_axis_richardson(const AArgs<T> &a, int n, const cx<T> *c1, double *err_part,
+                                              double *lp_part, hipStream_t s);
+int axis_richardson_tiles(int precision, int nt, int nx);
+/* sums of the n (a, b) pairs part[2i], part[2i+1] in a fixed order into out[0..1] */
+int launch_sum_pairs(const double *part, long n, double *out, hipStream_t s);
 template <class T> int launch_energy_partials(const cx<T> *f, long n, double *part, int nblocks, hipStream_t s);
-template <class T> int launch_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n, hipStream_t s);
 template <class T> int launch_ioniz(const cx<T> *f, int nt, long nlines, const PlasmaK &pk,
                                     cx<double> *tau_F, hipStream_t s);
 int launch_adk(const double *E, long n, const double *kon, double *W1, double *W2, hipStream_t s);
@@ -154,6 +160,8 @@ struct mpc_ctx_s {
     double *diag_line = nullptr;     /* per-line power */
     double *diag_out = nullptr;      /* reduced temp/spec */
     double *err_part = nullptr;
+    double *rich_err = nullptr;      /* adaptive: per-tile error norms of the fused sweep, then their sums */
+    double *rich_lp = nullptr;       /* adaptive: per-chunk line powers of C2 */
     int diag_nblocks = 0;
     mpc::PlasmaK pk{};
     std::map<std::string, void *> tables;   /* table cache keyed by kind+coef */

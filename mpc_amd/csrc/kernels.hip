@@ -1,9 +1,10 @@
 /*
  * kernels.hip -- auxiliary kernels of the split-step propagator (the fused
  * t-line sweep is in tline.hip, the x/y sweeps in axis.hip): layout
- * transposes, diagnostics reductions, the adaptive-step error and Richardson
- * sweeps, and the plasma/ADK kernels behind the legacy ioniz()/ADK() entry
- * points.
+ * transposes, diagnostics reductions, setStep's error norms (the legacy
+ * entry point; the adaptive driver fuses them into its last sweep), fixed-order
+ * sums, the guard's energy, and the plasma/ADK kernels behind the legacy
+ * ioniz()/ADK() entry points.
  */
 #include "device.h"
 
@@ -213,8 +214,8 @@ int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hi
 }
 
 /* ---------------------------------------------------------------------- */
-/* adaptive step: error partials and Richardson (propagation.c:625-631,    */
-/* MPC.c:377-381)                                                          */
+/* setStep's error norms (propagation.c:625-631), fixed-order sums, energy  */
+/*                                                                         */
 /* ---------------------------------------------------------------------- */
 template <class T>
 __global__ void __launch_bounds__(256) k_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part) {
@@ -241,6 +242,34 @@ template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, lon
 template int launch_err_partials<double>(const cx<double> *, const cx<double> *, long, double *, int, hipStream_t);
 template int launch_err_partials<float>(const cx<float> *, const cx<float> *, long, double *, int, hipStream_t);
 
+/* one block: sum of n (a, b) pairs in a fixed order */
+__global__ void __launch_bounds__(256) k_sum_pairs(const double *part, long n, double *out) {
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (long i = threadIdx.x; i < n; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1[0];
+        out[1] = s2[0];
+    }
+}
+int launch_sum_pairs(const double *part, long n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, s, part, n, out);
+    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+}
+
 /* per-block sum |f|^2 in a fixed order (the solver guard's energy) */
 template <class T>
 __global__ void __launch_bounds__(256) k_energy_partials(const cx<T> *f, long n, double *part) {
@@ -263,19 +292,6 @@ template <class T> int launch_energy_partials(const cx<T> *f, long n, double *pa
 template int launch_energy_partials<double>(const cx<double> *, long, double *, int, hipStream_t);
 template int launch_energy_partials<float>(const cx<float> *, long, double *, int, hipStream_t);
 
-template <class T> __global__ void k_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double r = (4.0 / 3.0) * (double)c2[i].re - (1.0 / 3.0) * (double)c1[i].re;
-        const double m = (4.0 / 3.0) * (double)c2[i].im - (1.0 / 3.0) * (double)c1[i].im;
-        a[i] = mk((T)r, (T)m);
-    }
-}
-template <class T> int launch_richardson(cx<T> *a, const cx<T> *c1, const cx<T> *c2, long n, hipStream_t s) {
-    hipLaunchKernelGGL((k_richardson<T>), dim3(2048), dim3(256), 0, s, a, c1, c2, n);
-    return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
-}
-template int launch_richardson<double>(cx<double> *, const cx<double> *, const cx<double> *, long, hipStream_t);
-template int launch_richardson<float>(cx<float> *, const cx<float> *, const cx<float> *, long, hipStream_t);
 
 /* ---------------------------------------------------------------------- */
 /* legacy ioniz(): tau_c and F per element (internal layout outputs)       */

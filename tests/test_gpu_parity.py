@@ -150,6 +150,44 @@ def test_adaptive_matches_oracle(gpu_lib):
     assert O.rel_l2(got, ref) < TOL64
 
 
+_adaptive_ref = {}
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+def test_adaptive_across_pass_ends_matches_oracle(gpu_lib, prec, tol):
+    """64x64x512, 4 mm cells with the curved mirror: 12 accepted steps that
+    cross 8 pass ends, with rejections, shrinking and growing steps (every
+    error ratio stays at least 2% from a decision threshold): the same
+    accepted count and dz sequence as the oracle, and the field"""
+    print(f"case adaptive 64x64x512 12 steps prec={prec}", flush=True)
+    wl = configs.small(64, 512, steps_per_pass=4, passes=12, mirror_R=0.3, energy=3e-4)
+    wl.def_err, wl.dzmin, wl.cell_L = 1e-3, 1e-7, 0.004
+    inp = make_inputs(wl)
+    A0 = inp.calibrated()
+    if "r" not in _adaptive_ref:
+        po = O.params_from(inp)
+        ref = A0.copy()
+        z, dz, sm = C.c_double(0.0), C.c_double(inp.params.dzz), C.c_int(1)
+        errs_o, dzs_o = np.zeros(12), np.zeros(12)
+        n_o = O.oracle().oracle_run_adaptive(O.vp(ref), C.byref(po), C.byref(z), C.byref(dz), C.byref(sm), 12,
+                                             O.dp(errs_o), O.dp(dzs_o))
+        _adaptive_ref["r"] = (ref, z.value, n_o, errs_o, dzs_o)
+    ref, z_o, n_o, errs_o, dzs_o = _adaptive_ref["r"]
+    assert n_o == 12 and z_o > 8 * wl.cell_L
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        r = p.run_adaptive(0.0, inp.params.dzz, 1, 12)
+        got = p.download()
+    err = O.rel_l2(got, ref)
+    print(f"  accepted {r['accepted']}/{n_o} z={r['z']:.6g}/{z_o:.6g} rel_l2={err:.3e} "
+          f"err ratios {np.round(r['errs'] / wl.def_err, 3)}", flush=True)
+    assert r["accepted"] == n_o
+    np.testing.assert_allclose(r["dzs"], dzs_o[:n_o], rtol=1e-12)
+    np.testing.assert_allclose(r["errs"], errs_o[:n_o], rtol=1e-6 if prec == "f64" else 2e-2)
+    assert abs(r["z"] - z_o) < 1e-15
+    assert err <= tol
+
+
 def test_cfg1_full_pass_fp64(gpu_lib):
     """64x64x512, one pass of 200 steps: field, energy, spectral width, peak rho."""
     print("case cfg1 200 steps fp64", flush=True)
