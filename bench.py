@@ -149,14 +149,12 @@ def against_oracle(inp, A_ref, nsteps, rho_ref):
     return out
 
 
-def batch_bench(steps, warmup):
-    """the 48-pulse batched workload in FP32: every pulse advances `steps`
-    z-steps in one batched call (one launch per sweep and group of same-size
-    pulses, a HIP graph per pass), against the same pulses run one after another.  The rate
-    is in 256x256x2048-equivalent steps/s: sum_p N_p * steps / N_ref per s.
-    `steps` must stay below the 300 steps of a pass (no mirror inside)."""
-    from mpc_amd import Batch, Propagator, configs, make_inputs
-    wls = configs.cfg4_batch()
+def _batch_vs_one_by_one(wls, steps, label):
+    """pulses advancing `steps` z-steps together (mpc_batch, one launch per
+    sweep and launch group, a HIP graph per pass) against the same pulses run
+    one after another with the same variants; `steps` must stay below a pass
+    (three calls of `steps` steps)"""
+    from mpc_amd import Batch, Propagator, make_inputs
     props, npts = [], []
     tuned = {}
     t0 = time.perf_counter()
@@ -173,10 +171,8 @@ def batch_bench(steps, warmup):
             p.set_variants(*tuned[key])
         props.append(p)
         npts.append(wl.nx * wl.nx * wl.nt)
-    log(f"[bench] batch: {len(props)} pulses set up in {time.perf_counter() - t0:.1f}s; "
+    log(f"[bench] {label}: {len(props)} pulses set up in {time.perf_counter() - t0:.1f}s; "
         f"grids {sorted(set((w.nx, w.nt) for w in wls))}; variants {tuned}")
-    nref = 256 * 256 * 2048
-    work = sum(npts) * steps / nref
     b = Batch(props)
     # the warm-up call has the timed call's launch list (same length, no
     # mirror inside), so the pass graph is captured before the timed region
@@ -184,23 +180,45 @@ def batch_bench(steps, warmup):
     t0 = time.perf_counter()
     b.run(steps, steps, graph=True)
     dt_b = time.perf_counter() - t0
-    log(f"[bench] batch: {steps} steps of all pulses in {dt_b:.3f}s ({work / dt_b:.1f} equivalent steps/s), "
-        f"graphs {b.graphs()}")
+    graphs = b.graphs()
     b.close()
     t0 = time.perf_counter()
     for p in props:
         p.run(2 * steps, steps)
     dt_s = time.perf_counter() - t0
-    log(f"[bench] one by one: {dt_s:.3f}s ({work / dt_s:.1f} equivalent steps/s)")
     for p in props:
         p.close()
+    nref = 256 * 256 * 2048
+    work = sum(npts) * steps / nref
+    log(f"[bench] {label}: batched {dt_b:.3f}s ({work / dt_b:.1f} equivalent steps/s, graphs {graphs}); "
+        f"one by one {dt_s:.3f}s ({work / dt_s:.1f}); ratio {dt_s / dt_b:.3f}")
+    return {"batched_equivalent_steps_per_s": round(work / dt_b, 3),
+            "one_by_one_equivalent_steps_per_s": round(work / dt_s, 3),
+            "batched_over_one_by_one": round(dt_s / dt_b, 4),
+            "pulse_steps_per_s_batched": round(len(wls) * steps / dt_b, 1),
+            "pulse_steps_per_s_one_by_one": round(len(wls) * steps / dt_s, 1),
+            "pulses": len(wls), "steps": steps, "points_total": int(sum(npts)),
+            "unit": "256x256x2048-equivalent steps/s", "dtype": "f32"}
+
+
+def batch_bench(steps, warmup):
+    """the 48-pulse batched workload in FP32 (rate in 256x256x2048-equivalent
+    steps/s: sum_p N_p * steps / N_ref per s), plus 48 small pulses
+    (64x64x256, the pass surrogate's solver workload) where launches
+    dominate a step"""
+    from mpc_amd import configs, surrogate
+    big = _batch_vs_one_by_one(configs.cfg4_batch(), steps, "batch")
+    small = _batch_vs_one_by_one(surrogate.pulse_workloads(48, seed=20261209, passes=10), min(3 * steps, 60),
+                                 "small-pulse batch")
     return {"metric": "batched split-step, 256x256x2048-equivalent steps/s (48 pulses, FP32)",
-            "value": round(work / dt_b, 3), "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": steps,
-            "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded pulses as drawn below)",
+            "value": big["batched_equivalent_steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": steps, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic (seeded pulses as drawn below)",
             "config": {"workload": "48 pulses drawn from {128^2x512, 192^2x1024, 256^2x2048}, 0.1-2 mJ, "
                                    "100-500 fs, Ar 0.5-3 bar, w0 0.5-1 mm; 10 passes x 300 steps",
-                       "points_total": int(sum(npts)), "one_by_one_steps_per_s": round(work / dt_s, 3),
-                       "graphs": True}}
+                       "one_by_one_steps_per_s": big["one_by_one_equivalent_steps_per_s"],
+                       "points_total": big["points_total"], "graphs": True, "cfg4": big,
+                       "small_pulses_64x64x256": small}}
 
 
 def load_traffic(prec):
@@ -367,10 +385,10 @@ def main(argv=None):
         br = batch_bench(bsteps, bsteps)
         batched = {"equivalent_steps_per_s": br["value"],
                    "one_by_one_equivalent_steps_per_s": br["config"]["one_by_one_steps_per_s"],
-                   "batched_over_one_by_one": round(br["value"] / br["config"]["one_by_one_steps_per_s"], 4),
+                   "batched_over_one_by_one": br["config"]["cfg4"]["batched_over_one_by_one"],
                    "steps": bsteps, "pulses": 48, "points_total": br["config"]["points_total"],
                    "workload": br["config"]["workload"], "unit": "256x256x2048-equivalent steps/s",
-                   "dtype": "f32"}
+                   "dtype": "f32", "small_pulses_64x64x256": br["config"]["small_pulses_64x64x256"]}
 
     out = {
         "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1,

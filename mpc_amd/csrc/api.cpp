@@ -532,6 +532,11 @@ struct BLaunch {
     const long *starts;
     int step;
     std::vector<int> members;   /* batch indices, for the per-field bookkeeping */
+    /* a launch group of one field runs the per-field kernel with this
+     * descriptor (the same arithmetic per line and tile, without the batched
+     * kernels' per-field lookups) */
+    std::vector<unsigned char> single;
+    int outer = 0;              /* 'x' / 'y': lines per t value of that field */
 };
 }
 
@@ -602,6 +607,10 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
             }
             starts.push_back(tot);                           /* starts[n] = total */
             L.variant = b->ctx[k.m[0]]->tvar;                 /* persistent (0) or per group (1) */
+            if (d.size() == 1) {
+                L.single.resize(sizeof(TArgs<T>));
+                std::memcpy(L.single.data(), &d[0], sizeof(TArgs<T>));
+            }
             L.desc = pool_put(b, d.data(), d.size() * sizeof(TArgs<T>));
         } else {
             const int axis = kind == 'x' ? 0 : 1;
@@ -625,11 +634,19 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
                 tot += (long)nch * a.n_outer;
             }
             starts.push_back(tot);                           /* starts[n] = total */
+            if (d.size() == 1) {
+                L.single.resize(sizeof(AArgs<T>));
+                std::memcpy(L.single.data(), &d[0], sizeof(AArgs<T>));
+                L.outer = d[0].n_outer;
+            }
             L.desc = pool_put(b, d.data(), d.size() * sizeof(AArgs<T>));
         }
         L.starts = (const long *)pool_put(b, starts.data(), starts.size() * sizeof(long));
         L.nblocks = tot;
-        L.grid = kind == 't' ? tline_batch_grid<T>(tot, k.len, k.plasma != 0, L.variant) : tot;
+        if (!L.single.empty())
+            L.grid = kind == 't' ? tline_grid<T>(b->ctx[k.m[0]]->nxy, k.len, k.plasma != 0, L.variant) : tot;
+        else
+            L.grid = kind == 't' ? tline_batch_grid<T>(tot, k.len, k.plasma != 0, L.variant) : tot;
         if (!L.desc || !L.starts) return MPC_ENOMEM;
         out.push_back(std::move(L));
     }
@@ -637,6 +654,16 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
 }
 
 template <class T> static int batch_launch(mpc_batch *b, const BLaunch &L) {
+    if (!L.single.empty()) {
+        if (L.kind == 't') {
+            TArgs<T> a;
+            std::memcpy(&a, L.single.data(), sizeof a);
+            return launch_tline<T>(a, L.len, L.plasma != 0, L.variant, b->stream, nullptr);
+        }
+        AArgs<T> a;
+        std::memcpy(&a, L.single.data(), sizeof a);
+        return launch_axis<T>(a, L.len, L.outer, L.variant, b->stream);
+    }
     if (L.kind == 't')
         return launch_tline_batch<T>((const TArgs<T> *)L.desc, L.starts, L.n, L.nblocks, L.len, L.plasma != 0,
                                      L.variant, b->stream);

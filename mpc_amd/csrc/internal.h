@@ -123,6 +123,8 @@ int tline_lines_per_block(int nt, bool plasma, int prec);
 template <class T>
 int launch_tline_batch(const TArgs<T> *d, const long *gstart, int n, long total, int nt, bool plasma, int variant,
                        hipStream_t s);
+/* workgroups a per-field t-line launch over nlines lines runs (its plasma slots) */
+template <class T> long tline_grid(long nlines, int nt, bool plasma, int variant);
 /* workgroups a batched t-line launch over `total` line groups runs (variant 0: persistent) */
 template <class T> long tline_batch_grid(long total, int nt, bool plasma, int variant);
 template <class T>

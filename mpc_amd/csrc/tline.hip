@@ -330,17 +330,23 @@ k_tline_batch_p(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart,
 bool tline_supported(int nt) { return nt == 256 || nt == 512 || nt == 1024 || nt == 2048 || nt == 4096; }
 int tline_variants(void) { return 2; }
 
-template <class T, class G, bool PL, bool PERSIST>
-static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
+/* workgroups a per-field launch over ngroups line groups runs: all of them,
+ * or for the persistent form what fits on the device at once */
+template <class T, class G, bool PL, bool PERSIST> static long tline_grid_of(long ngroups) {
     using C = TCfg<T, G, PL>;
-    if (a.nlines % C::LPB) return MPC_EINVAL;
-    const long ngroups = a.nlines / C::LPB;
     long nb = ngroups;
     if (PERSIST) {
         static long resident = -1;              /* per instantiation: the same on every device of one model */
         if (resident < 0) resident = resident_blocks((const void *)k_tline<T, G, PL, PERSIST>, C::THREADS, 0);
         if (resident > 0 && resident < nb) nb = resident;
     }
+    return nb;
+}
+template <class T, class G, bool PL, bool PERSIST>
+static int tline_go(const TArgs<T> &a, hipStream_t s, int *nblocks) {
+    using C = TCfg<T, G, PL>;
+    if (a.nlines % C::LPB) return MPC_EINVAL;
+    const long nb = tline_grid_of<T, G, PL, PERSIST>(a.nlines / C::LPB);
     if (nb <= 0 || nb > 0x7fffffffL) return MPC_EINVAL;
     hipLaunchKernelGGL((k_tline<T, G, PL, PERSIST>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a);
     if (nblocks) *nblocks = (int)nb;
@@ -429,6 +435,25 @@ template <class T> int launch_tline(const TArgs<T> &a, int nt, bool plasma, int 
 }
 template int launch_tline<double>(const TArgs<double> &, int, bool, int, hipStream_t, int *);
 template int launch_tline<float>(const TArgs<float> &, int, bool, int, hipStream_t, int *);
+
+template <class T, int NT> static long tline_grid_t(long nlines, bool plasma, int variant) {
+    using GEO = TGeo<NT>;
+    const long ng = nlines / GEO::LPB;
+    if (variant == 0) return plasma ? tline_grid_of<T, GEO, true, true>(ng) : tline_grid_of<T, GEO, false, true>(ng);
+    return ng;
+}
+template <class T> long tline_grid(long nlines, int nt, bool plasma, int variant) {
+    switch (nt) {
+    case 256:  return tline_grid_t<T, 256>(nlines, plasma, variant);
+    case 512:  return tline_grid_t<T, 512>(nlines, plasma, variant);
+    case 1024: return tline_grid_t<T, 1024>(nlines, plasma, variant);
+    case 2048: return tline_grid_t<T, 2048>(nlines, plasma, variant);
+    case 4096: return tline_grid_t<T, 4096>(nlines, plasma, variant);
+    default: return 0;
+    }
+}
+template long tline_grid<double>(long, int, bool, int);
+template long tline_grid<float>(long, int, bool, int);
 
 /* t-line kernel block geometry (lines per workgroup; every variant's) */
 int tline_lines_per_block(int nt, bool plasma, int prec) {
