@@ -68,11 +68,15 @@ constexpr double ce_cos2pi(int m, int N) {
     if (4 * m > N) return -ce_cos_poly(ce_pi - 2.0 * ce_pi * m / N);
     return ce_cos_poly(2.0 * ce_pi * m / N);
 }
+/* not recursive, so that device code folds it to a constant instead of
+ * calling it (a call gives the kernel a stack in scratch memory) */
 constexpr double ce_sin2pi(int m, int N) {
     m %= N; if (m < 0) m += N;
-    if (2 * m > N) return -ce_sin2pi(N - m, N);
-    if (4 * m > N) return ce_sin_poly(ce_pi - 2.0 * ce_pi * m / N);
-    return ce_sin_poly(2.0 * ce_pi * m / N);
+    /* sin(2 pi - a) = -sin a */
+    const double sgn = 2 * m > N ? -1.0 : 1.0;
+    if (2 * m > N) m = N - m;
+    if (4 * m > N) return sgn * ce_sin_poly(ce_pi - 2.0 * ce_pi * m / N);
+    return sgn * ce_sin_poly(2.0 * ce_pi * m / N);
 }
 template <int DIR, class T> MPC_HD cx<T> ctw(int m, int N) {
     return mk((T)ce_cos2pi(m, N), (T)(DIR * ce_sin2pi(m, N)));
