@@ -2,14 +2,17 @@
  * fno.hip -- kernels of the Fourier-neural-operator pass surrogate.
  *
  * The reference has no learned part.  The surrogate maps the field at one
- * mirror plane to the field after a pass; every dense contraction in it
- * (the truncated DFTs along t, y and x, the per-mode channel mixing, the
- * pointwise channel maps, and all their transposes in the backward pass) is
- * a batched GEMM run here on the matrix cores: operands are rounded to BF16
- * as they are staged into LDS, products accumulate in FP32
- * (v_mfma_f32_16x16x16_bf16).  The rest are small elementwise kernels:
- * complex recombination after the real-block DFT products, bias + GELU and
- * its derivative, bias reductions and the Adam update.
+ * mirror plane to the field after a pass.  On the matrix cores: the
+ * truncated DFTs along t, y and x, their inverses and transposes, and the
+ * weight gradients of the 1x1 channel maps, as batched GEMMs (k_gemm_bf16):
+ * operands are rounded to BF16 as they are staged into LDS, products
+ * accumulate in FP32 (v_mfma_f32_16x16x16_bf16).  On the vector units, in
+ * FP32: the pointwise 1x1 channel maps with bias, residual and GELU fused
+ * (k_pointwise; at <= 64 channels a point is a few hundred FMAs against its
+ * traffic, so a matrix-core tiling would only add staging), the per-mode
+ * complex channel mixing and its backward (k_mix, k_mix_bwd), complex
+ * recombination after the real-block DFT products, bias reductions, the loss
+ * and the Adam update.
  */
 #include "internal.h"
 #include "../../include/mpc_fno.h"
