@@ -178,13 +178,39 @@ template <class G> __device__ __forceinline__ int padp_tm(int tid, int m) {
     return padi_strided<G::PAD, G::NT / G::E>(tid, m);
 }
 
+/* Knuth's TwoSum: s + e == a + b exactly, s = fl(a + b) */
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+/*
+ * Composes an affine segment map (gm, gc + gl) with the map of the segment
+ * that follows it in the scan, (om, oc + ol): R -> gm (om R + oc + ol) + gc +
+ * gl.  The constant term is kept as an unevaluated sum hi + lo (a
+ * compensated carry): the product's rounding error comes from an FMA, the
+ * sum's from TwoSum, and both go into lo.
+ */
+__device__ __forceinline__ void affine_combine(double &gm, double &gc, double &gl, double om, double oc, double ol) {
+    const double p = gm * oc;
+    const double ep = fma(gm, oc, -p);
+    double sh, es;
+    two_sum(p, gc, sh, es);
+    gl = gl + fma(gm, ol, ep + es);
+    gc = sh;
+    gm = gm * om;
+}
+
 /*
  * Reverse-t affine scan over one line.  Element k carries the map
  * f_k(R) = m_k R + c_k, given by coef(k, m_k, c_k); the result
  * R_k = f_k(R_{k+1}), R_NT = 0, is handed to out(k, R_k).  Thread tid owns the
  * consecutive segment [tid*E, tid*E + E): segment maps are composed serially,
  * combined across lanes with a shuffle-based suffix scan and across wavefronts
- * through LDS (scr), then each thread re-walks its segment.  coef must give
+ * through LDS (scr, 3 NW + TPL doubles), then each thread re-walks its
+ * segment from the suffix value of the segments after it.  The carries of
+ * the lane and wavefront combines are compensated (affine_combine); the
+ * suffix value is rounded once, where a thread's walk starts.  coef must give
  * the same values on both walks; out may overwrite what coef read at k.
  */
 template <class G, class Coef, class Out>
@@ -193,7 +219,7 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
     constexpr int W = TPL < 64 ? TPL : 64;
     constexpr int NW = (TPL + 63) / 64;
     const int lo = tid * E;
-    double gm = 1.0, gc = 0.0;
+    double gm = 1.0, gc = 0.0, gl = 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
         double mk, ck;
@@ -206,24 +232,18 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
     for (int d = 1; d < W; d <<= 1) {
         const double om = __shfl_down(gm, d, W);
         const double oc = __shfl_down(gc, d, W);
-        if (lane + d < W) {
-            gc = gm * oc + gc;
-            gm = gm * om;
-        }
+        const double ol = __shfl_down(gl, d, W);
+        if (lane + d < W) affine_combine(gm, gc, gl, om, oc, ol);
     }
     if constexpr (NW > 1) {
         const int wv = tid / 64;
-        if (lane == 0) { scr[2 * wv] = gm; scr[2 * wv + 1] = gc; }
+        if (lane == 0) { scr[3 * wv] = gm; scr[3 * wv + 1] = gc; scr[3 * wv + 2] = gl; }
         __syncthreads();
-        for (int w2 = wv + 1; w2 < NW; ++w2) {
-            const double om = scr[2 * w2], oc = scr[2 * w2 + 1];
-            gc = gm * oc + gc;
-            gm = gm * om;
-        }
+        for (int w2 = wv + 1; w2 < NW; ++w2) affine_combine(gm, gc, gl, scr[3 * w2], scr[3 * w2 + 1], scr[3 * w2 + 2]);
     }
-    scr[2 * NW + tid] = gc;
+    scr[3 * NW + tid] = gc + gl;
     __syncthreads();
-    double R = (tid + 1 < TPL) ? scr[2 * NW + tid + 1] : 0.0;
+    double R = (tid + 1 < TPL) ? scr[3 * NW + tid + 1] : 0.0;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
         double mk, ck;
@@ -241,7 +261,7 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
 template <class G, class T> struct PlasmaLds {
     static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
     static constexpr int ARR = padded_elems<G::PAD>(NT);   /* elements per array */
-    static constexpr int SCR = 2 * NW + TPL;                /* doubles */
+    static constexpr int SCR = 3 * NW + TPL;                /* doubles: the scan's scratch */
     static constexpr int SVB = 2 * ARR * (int)sizeof(T);
     static constexpr bool SV = G::LPB * ((2 * ARR + SCR) * 8 + SVB) <= 80 * 1024;
     static constexpr int BYTES = (2 * ARR + SCR) * 8 + (SV ? SVB : 0);
