@@ -216,3 +216,60 @@ def test_adam_training_lowers_the_loss(gpu_lib):
         last = net.loss_and_grad(X, T)
     print(f"case fno adam: loss {first:.4f} -> {last:.4f}", flush=True)
     assert last <= 0.5 * first
+
+
+def _configured():
+    """the configured network (64x64x256, width 32, 4 layers, 16x16x32
+    modes) on a smooth input whose modes all lie inside the kept window, with
+    the spectral weights scaled so the spectral path carries about as much as
+    the 1x1 path"""
+    from mpc_amd.fno import FnoConfig, init_params
+    import scipy.fft
+    cfg = FnoConfig()
+    p = init_params(cfg, seed=4)
+    for l in range(cfg.layers):
+        p[f"Rr{l}"] *= 300
+        p[f"Ri{l}"] *= 300
+    rng = np.random.default_rng(21)
+    spec = np.zeros((1, 2, cfg.X, cfg.Y, cfg.T // 2 + 1), dtype=np.complex128)
+    for kx in list(range(0, 6)) + list(range(cfg.X - 5, cfg.X)):
+        for ky in list(range(0, 6)) + list(range(cfg.Y - 5, cfg.Y)):
+            spec[:, :, kx, ky, :12] = rng.standard_normal((1, 2, 12)) + 1j * rng.standard_normal((1, 2, 12))
+    f = scipy.fft.irfftn(spec, s=(cfg.X, cfg.Y, cfg.T), axes=(2, 3, 4))
+    f /= f.std()
+    x = np.concatenate([f, np.full((1, 1, cfg.X, cfg.Y, cfg.T), 0.3), np.full((1, 1, cfg.X, cfg.Y, cfg.T), -0.5)], axis=1)
+    tgt = 0.8 * x[:, :2] + 0.1 * np.roll(x[:, :2], 3, axis=-1)
+    return cfg, p, x, tgt
+
+
+def test_configured_network_matches_cpu_network(gpu_lib):
+    """64x64x256, width 32: forward against the FP64 CPU network (BF16
+    tolerance on the change the network makes), and the analytic gradient
+    against central finite differences of the CPU loss along two random
+    directions: all parameters at once, and the last layer's spectral
+    weights alone"""
+    from mpc_amd.fno import FnoGpu
+    cfg, p, x, tgt = _configured()
+    net = FnoGpu(cfg, p)
+    got = net.forward(_t(x)).numpy()
+    print("case fno configured size: CPU forward", flush=True)
+    ref = fno_cpu.forward(p, cfg, x)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref - x[:, :2])
+    print(f"  forward rel err of the change {err:.2e}", flush=True)
+    assert err <= 3e-2
+    loss_gpu = net.loss_and_grad(_t(x), _t(tgt))
+    grads = net.grads()
+    rng = np.random.default_rng(23)
+    for label, keys in (("all parameters", list(p)), ("Rr3, Ri3", ["Rr3", "Ri3"])):
+        d = {k: rng.standard_normal(p[k].shape) * (float(np.abs(p[k]).std()) or 1.0) for k in keys}
+        eps = 1e-4
+        pp = {k: (v + eps * d[k] if k in d else v) for k, v in p.items()}
+        pm = {k: (v - eps * d[k] if k in d else v) for k, v in p.items()}
+        lp = fno_cpu.loss(pp, cfg, x, tgt)
+        lm = fno_cpu.loss(pm, cfg, x, tgt)
+        fd = (lp - lm) / (2 * eps)
+        an = float(sum(np.sum(grads[k] * d[k]) for k in keys))
+        scale = np.sqrt(sum(float(np.sum((grads[k] * np.abs(p[k]).std()) ** 2)) for k in keys))
+        rel = abs(an - fd) / max(abs(fd), scale, 1e-12)
+        print(f"  {label}: loss gpu {loss_gpu:.6f} fd {fd:+.6e} gpu {an:+.6e} rel {rel:.2e}", flush=True)
+        assert rel <= 5e-2, label
