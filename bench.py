@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark: split-step steps/s of the 256x256x2048 argon workload on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-    python bench.py --batch        (the 48-pulse batched workload instead)
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batch]
+    python bench.py --batch        (the batched workloads alone)
     python bench.py --surrogate    (the pass surrogate experiment instead)
     python bench.py --adaptive     (adaptive step doubling + Richardson instead)
 
@@ -13,10 +13,17 @@ path (FP64 kept for the propagator phase, the ADK exponent and the density
 scan); the plain FP64 path is timed on the same inputs, and the FP32 field is
 compared with the FP64 one after the same steps (rel L2).
 
+The default run also
+* compares the GPU's FP64 and FP32 fields, energies, rms spectral widths and
+  peak densities with the FP64 CPU oracle's after the steps its cpu_baseline
+  leg ran (config.vs_oracle), and
+* times the 48-pulse batched workload and 48 small (64x64x256) pulses, batched
+  against one by one (config.batched).
+
 Prints ONE JSON line.  The roofline object is for the dominant sweep kernel,
 timed with HIP events on the library's stream during a separate pass over the
 same steps; cpu_baseline is the FP64 CPU oracle on a bounded sample of the
-same workload.
+same workload (with the host's core count and CPU model).
 """
 from __future__ import annotations
 
