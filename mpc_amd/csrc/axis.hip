@@ -271,12 +271,13 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
  *   lp_part[chunk][y][x]   = sum over the chunk's t of |C2|^2 (FP64, t order)
  * and stores R = 4/3 C2 - 1/3 C1 (formed in FP64) in place of C2, so an
  * accepted step only has to adopt that buffer.  One workgroup per tile,
- * chunks of each x consecutive; the plan is the 8-per-thread one where its
- * workgroup fits in 1024 threads, else Plan<N>.
+ * chunks of each x consecutive; the plan is Plan<N> (radix 16 where it
+ * applies) where its workgroup fits in 1024 threads, else the 8-per-thread
+ * one.
  */
 template <class T, int N> struct RichCfg {
     static constexpr int G = ATile<T>::G;
-    using PL = std::conditional_t<(G * (N / APlan8<N>::E) <= 1024), APlan8<N>, Plan<N>>;
+    using PL = std::conditional_t<(G * (N / Plan<N>::E) <= 1024), Plan<N>, APlan8<N>>;
     using C = ACfg<T, N, PL, G>;
 };
 template <class T, int N>
@@ -287,7 +288,7 @@ k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__
     using PL = typename RC::PL;
     using C = typename RC::C;
     constexpr int E = C::E, TPL = C::TPL, G = C::G, THREADS = C::THREADS;
-    __shared__ __align__(16) cx<T> lds[N * G];
+    __shared__ __align__(16) cx<T> lds[N * G + G];   /* G more: the padded line-power rows below */
     constexpr int W = THREADS < 64 ? THREADS : 64, NWV = THREADS / W;   /* lanes per wave, waves */
     __shared__ double wred[2 * NWV];
     const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
@@ -306,7 +307,13 @@ k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__
     };
     axis_apply<T, N, PL, G, false>(a, v, p, outer, tid, Dc, fft);
     __syncthreads();                                  /* the transform's last LDS reads are done */
-    double *pw_t = reinterpret_cast<double *>(lds);   /* [line along y][t in chunk] */
+    /* |C2|^2 per (t in chunk, line along y), rows padded to N + 1 doubles so
+     * that both the writes (consecutive lanes: consecutive t) and the reads
+     * (consecutive lanes: consecutive lines) are free of bank conflicts */
+    constexpr int PW = N + 1;
+    static_assert(G * PW * (int)sizeof(double) <= (N * G + G) * (int)sizeof(cx<T>),
+                  "line-power scratch must fit in the transform's LDS");
+    double *pw_t = reinterpret_cast<double *>(lds);
     double e1 = 0.0, e2 = 0.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -316,7 +323,7 @@ k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__
         const double dr = cr - ur, di = ci - ui, pw = cr * cr + ci * ci;
         e1 += dr * dr + di * di;
         e2 += pw;
-        pw_t[(tid + TPL * m) * G + gcol] = pw;
+        pw_t[gcol * PW + tid + TPL * m] = pw;
         gstore(a.field + off0, off, mk((T)((4.0 / 3.0) * cr - (1.0 / 3.0) * ur), (T)((4.0 / 3.0) * ci - (1.0 / 3.0) * ui)));
     }
     /* block sums of the two error norms: waves, then the wave totals in order */
@@ -339,7 +346,7 @@ k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__
     const long chunk = t0 / G;
     for (int j = threadIdx.x; j < N; j += THREADS) {
         double sum = 0.0;
-        for (int g = 0; g < G; ++g) sum += pw_t[j * G + g];
+        for (int g = 0; g < G; ++g) sum += pw_t[g * PW + j];
         lp_part[chunk * nxy + (long)j * nx + outer] = sum;
     }
 }

@@ -64,9 +64,26 @@ def test_driver_adaptive_runs(gpu_lib, tmp_path):
     base = str(tmp_path / "data")
     rundir = write_inputs(inp, base, 0)
     out = _run(["-d", base, "-n", "0", "-m", "4"], str(tmp_path))
-    assert "local error" in out or "We are here" in out
+    assert "We are here" in out
     lines = open(os.path.join(rundir, "other.bin")).read().strip().splitlines()
     assert len(lines) == 4 and all("\t" in l for l in lines)
+    # the driver's 4 accepted steps against the oracle's adaptive run: the
+    # field, and err / dz as other.bin records them ("%f\t%f")
+    import ctypes as C
+    ref = inp.calibrated()
+    po = O.params_from(inp)
+    z, dz, sm = C.c_double(0.0), C.c_double(inp.params.dzz), C.c_int(1)
+    errs, dzs = np.zeros(4), np.zeros(4)
+    n = O.oracle().oracle_run_adaptive(O.vp(ref), C.byref(po), C.byref(z), C.byref(dz), C.byref(sm), 4,
+                                       O.dp(errs), O.dp(dzs))
+    assert n == 4
+    got = np.fromfile(os.path.join(rundir, "out.bin")).view(np.complex128).reshape(ref.shape)
+    err = O.rel_l2(got, ref)
+    print(f"  driver adaptive vs oracle: rel_l2={err:.3e}", flush=True)
+    assert err < 1e-10
+    for line, e, d in zip(lines, errs, dzs):
+        le, ld = (float(v) for v in line.split("\t"))
+        assert le == float(f"{e:f}") and ld == float(f"{d:f}")
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])

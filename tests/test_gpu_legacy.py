@@ -145,3 +145,66 @@ def test_setstep_matches_oracle(gpu_lib):
                                   C.byref(bado), C.byref(erro))
         assert abs(err.value - erro.value) < 1e-13 * erro.value
         assert bad.value == bado.value and dz.value == dzo.value
+
+
+def test_create_plans_writes_and_reads_the_variant_cache(gpu_lib, tmp_path):
+    """createPlans' cache file plays FFTW wisdom's role (propagation.c:540-611):
+    with no file the first ssfmprop tunes the kernel variants and writes
+    "<cpath><Nx>"; a later createPlans(readin=1) reads it and the step runs
+    those variants without tuning again (bit-identical to a context set to
+    them); an unreadable file is reported and replaced"""
+    from mpc_amd import Propagator
+    print("case legacy createPlans cache 32x32x512", flush=True)
+    inp = make_inputs(configs.small(32, 512, mirror_R=0.0))
+    A = inp.calibrated()
+    P = inp.params
+    dims = O.dims_arr(A.shape)
+    z = 0.3 * P.Cav
+    cpath = str(tmp_path / "wisdom_")
+    path = cpath + "32"
+    L = lib()
+
+    def step_with_cache():
+        hs = [C.c_void_p() for _ in range(6)]
+        L.createPlans(1, O.ip(dims), cpath.encode(), *[C.byref(h) for h in hs])
+        got = A.copy()
+        sm, pe = C.c_int(1), C.c_double(P.puls_e)
+        bg = [np.zeros(32), np.zeros(32), np.zeros(4), np.zeros(4)]
+        b1, b2 = C.c_double(0), C.c_double(0)
+        L.ssfmprop(O.vp(got), O.ip(dims), P.sst_on, P.dzz, O.dp(inp.betas), P.alpha, C.byref(sm), z, P.Cav, P.k0,
+                   O.dp(inp.wr), *hs, O.dp(inp.w), P.gamma2, P.w0, P.plasm, P.deltat, P.gas, P.rho_c, P.rho_nt,
+                   P.n0, C.byref(pe), O.dp(inp.r), *[O.dp(b) for b in bg], C.byref(b1), C.byref(b2), P.Ab_M, 0, 0)
+        _free(hs)
+        assert L.mpc_last_error() == 0
+        return got
+
+    import os
+    import re
+    assert not os.path.exists(path)
+    first = step_with_cache()                         # tunes, then writes the cache
+    text = open(path).read()
+    m = re.match(r"mpc-kernel-variants tline=(\d+) axis=(\d+) yaxis=(\d+) nx=32 ny=32 nt=512", text)
+    assert m, text
+    tv, xv, yv = map(int, m.groups())
+    assert 0 <= tv < 2 and 0 <= xv < 32 and 0 <= yv < 32
+    # a cache with other variants is read and used as it is
+    open(path, "w").write("mpc-kernel-variants tline=1 axis=2 yaxis=27 nx=32 ny=32 nt=512\n")
+    cached = step_with_cache()
+    assert open(path).read().startswith("mpc-kernel-variants tline=1 axis=2 yaxis=27")   # not re-tuned
+    with Propagator(inp, "f64") as p:
+        p.upload(A)
+        p.set_variants(1, 2, 27)
+        p.step(P.dzz, z, 1)
+        same = p.download()
+    assert np.array_equal(cached, same)
+    ref = A.copy()
+    po = O.params_from(inp)
+    sm_o, pe_o = C.c_int(1), C.c_double(P.puls_e)
+    bo = [np.zeros(32), np.zeros(32), np.zeros(4), np.zeros(4)]
+    O.oracle().oracle_ssfmprop(O.vp(ref), C.byref(po), P.dzz, C.byref(sm_o), z, C.byref(pe_o),
+                               *[O.dp(b) for b in bo], C.byref(C.c_double()), C.byref(C.c_double()), 0)
+    assert O.rel_l2(first, ref) < 1e-10 and O.rel_l2(cached, ref) < 1e-10
+    # an unreadable cache is replaced by a tuned one
+    open(path, "w").write("not a cache\n")
+    step_with_cache()
+    assert open(path).read().startswith("mpc-kernel-variants tline=")
