@@ -95,6 +95,8 @@ static int build_pk(mpc_ctx *c) {
     k.K4 = k0 / (p.n0 * p.rho_c) * p.w0;
     k.K5 = p.w0 * p.w0;
     k.w0 = p.w0;
+    /* sigma = K4 tau / (1 + K5 tau^2) <= K4 / (2 sqrt K5) and sigma tau <= K4 / K5 */
+    k.sig_bound = k.K4 / (2.0 * std::sqrt(k.K5)) + p.w0 * k.K4 / k.K5;
     k.ve_fac = std::sqrt(k.K1 / k.K2) / p.w0;
     k.I1e = I1 * EL0;
     k.Kon = p.deltat / T0;

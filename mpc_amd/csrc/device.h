@@ -178,6 +178,37 @@ template <class G> __device__ __forceinline__ int padp_tm(int tid, int m) {
     return padi_strided<G::PAD, G::NT / G::E>(tid, m);
 }
 
+/*
+ * Lane moves within a wavefront by DPP (no LDS round trip): CTRL is
+ * row_shr:d (0x110 + d, lanes of a 16-lane row take lane - d of the same row),
+ * row_bcast:15 (0x142, the next row takes lane 15 of a row) or row_bcast:31
+ * (0x143, rows 2 and 3 take lane 31); ROWS masks the rows written.  Lanes with
+ * no source, or in rows not written, return old.
+ */
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_move(double x, double old) {
+    const int2 v = __builtin_bit_cast(int2, x), o = __builtin_bit_cast(int2, old);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(o.x, v.x, CTRL, ROWS, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(o.y, v.y, CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, r);
+}
+/* sum of s over the W lanes of a group: W = 64, a whole wavefront by DPP
+ * (inclusive prefix sums, the total in lane 63); W < 64, butterfly shuffles
+ * (the total in every lane of the group) */
+template <int W> __device__ __forceinline__ void lanes_sum(double &s) {
+    if constexpr (W == 64) {
+        s += dpp_move<0x111, 0xf>(s, 0.0);
+        s += dpp_move<0x112, 0xf>(s, 0.0);
+        s += dpp_move<0x114, 0xf>(s, 0.0);
+        s += dpp_move<0x118, 0xf>(s, 0.0);
+        s += dpp_move<0x142, 0xa>(s, 0.0);
+        s += dpp_move<0x143, 0xc>(s, 0.0);
+    } else {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, W);
+    }
+}
+
 /* Knuth's TwoSum: s + e == a + b exactly, s = fl(a + b) */
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
     s = a + b;
@@ -213,19 +244,40 @@ __device__ __forceinline__ void affine_combine(double &gm, double &gc, double &g
  * suffix value is rounded once, where a thread's walk starts.  coef must give
  * the same values on both walks; out may overwrite what coef read at k.
  */
-template <class G, class Coef, class Out>
-__device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef coef, Out out) {
+template <class G, bool GATED = false, class Coef, class Out, class Gate = int>
+__device__ __forceinline__ bool affine_suffix_scan(int tid, double *scr, Coef coef, Out out,
+                                                   double gate_init = 0.0, Gate gate = 0) {
     constexpr int E = G::E, TPL = G::NT / E;
     constexpr int W = TPL < 64 ? TPL : 64;
     constexpr int NW = (TPL + 63) / 64;
     const int lo = tid * E;
-    double gm = 1.0, gc = 0.0, gl = 0.0;
+    double gm = 1.0, gc = 0.0, gl = 0.0, cs = gate_init;
 #pragma unroll
     for (int i = E - 1; i >= 0; --i) {
         double mk, ck;
         coef(lo + i, mk, ck);
         gc = mk * gc + ck;
         gm = mk * gm;
+        if constexpr (GATED) cs += ck;
+    }
+    if constexpr (GATED) {
+        /* the line's sum of c_k (plus gate_init of every thread) goes to
+         * gate(); when it returns true on every line of the workgroup the
+         * scan stops here and returns false (out is not called) */
+        double *part = scr + 3 * NW + TPL;                 /* NW doubles */
+        lanes_sum<W>(cs);
+        if ((tid % W) == (W == 64 ? 63 : 0)) part[tid / W] = cs;
+        __syncthreads();
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += part[w];
+        bool stop = gate(sum);
+        if constexpr (G::LPB > 1) {
+            stop = !__syncthreads_or(!stop);                /* whole workgroup or none */
+        } else {
+            stop = __builtin_amdgcn_readfirstlane((int)stop);   /* the same on every lane */
+        }
+        if (stop) return false;
     }
     const int lane = tid % W;
 #pragma unroll
@@ -252,6 +304,7 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
         out(lo + i, R);
     }
     __syncthreads();
+    return true;
 }
 
 /* per-line plasma LDS: U and X (FP64) and the scan's scratch, plus S (sigma)
@@ -261,11 +314,30 @@ __device__ __forceinline__ void affine_suffix_scan(int tid, double *scr, Coef co
 template <class G, class T> struct PlasmaLds {
     static constexpr int NT = G::NT, E = G::E, TPL = NT / E, NW = (TPL + 63) / 64;
     static constexpr int ARR = padded_elems<G::PAD>(NT);   /* elements per array */
-    static constexpr int SCR = 3 * NW + TPL;                /* doubles: the scan's scratch */
+    static constexpr int SCR = 3 * NW + TPL + NW;           /* doubles: the scan's scratch, then the
+                                                               cold-line test's per-wave partials */
     static constexpr int SVB = 2 * ARR * (int)sizeof(T);
     static constexpr bool SV = G::LPB * ((2 * ARR + SCR) * 8 + SVB) <= 80 * 1024;
     static constexpr int BYTES = (2 * ARR + SCR) * 8 + (SV ? SVB : 0);
 };
+
+/*
+ * Cold lines.  The first scan below is R_k = R_{k+1} (1 - a_k) + X_k with
+ * X_k = b_k (1 - a_k) + a_k >= a_k >= 0, so with S = sum_k X_k <= 1 over a
+ * line, R_{k+1} <= R_k <= R_0 <= S and R_0 >= S (1 - S).  Without the second
+ * species (c = 0) the ionised fraction is u_k = 1 - exp(-R_k) <= S, and the
+ * line's maximum u_0 >= S (1 - 3S/2).  The plasma part of the exponent,
+ * dz rho_nt/2 u sigma (1 + i w0 tau_c), is then at most
+ * |dz| rho_nt/2 S sigma_bound in modulus at every element, with
+ * sigma_bound >= sigma (1 + w0 tau) for every tau.  A line where that is
+ * below EPS (the field changes by less than that, relative: under the
+ * rounding of T) and S <= S_MAX skips the rest of the scans: its plasma part
+ * is taken as zero and its ionised-fraction maximum as S (relative error
+ * below 1.5 S_MAX).
+ */
+template <class T> struct ColdLine;
+template <> struct ColdLine<double> { static constexpr double S_MAX = 0x1p-42, EPS = 0x1p-53; };
+template <> struct ColdLine<float> { static constexpr double S_MAX = 0x1p-24, EPS = 0x1p-26; };
 
 /*
  * The per-element plasma quantities from |A|^2 (a2, the field scale folded
@@ -351,9 +423,10 @@ template <> struct PlasmaElem<float> {
  * FP64 in both precisions.
  * sink(m, F, tau) receives the result per element, in precision T.
  */
-template <class T, class G, class Sink>
+template <class T, class G, bool COLD = false, class Sink>
 __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, double *lds,
-                                            const PlasmaK &pk, double &rho_max, Sink sink) {
+                                            const PlasmaK &pk, double &rho_max, Sink sink,
+                                            double dz_abs = 0.0) {
     using L = PlasmaLds<G, T>;
     constexpr int NT = G::NT;
     constexpr int E = L::E, TPL = L::TPL;
@@ -392,10 +465,25 @@ __device__ __forceinline__ void plasma_line(const cx<T> (&A)[G::E], int tid, dou
         return pe;
     };
     __syncthreads();
-    /* U <- u = 1 - exp(-R) */
-    affine_suffix_scan<G>(tid, scr,
+    /* U <- u = 1 - exp(-R); a cold line (COLD) stops after the sum of X */
+    double s_line = 0.0;
+    double s_lim = 0.0;
+    if constexpr (COLD) {
+        /* the bound |dz| rho_nt/2 S sigma_bound taken twice over, for the
+         * rounding of sigma and tau in T; fmin drops the NaN of 0 * inf
+         * (no plasma part then) */
+        s_lim = fmin(ColdLine<T>::S_MAX, ColdLine<T>::EPS / (pk.rho_nt * dz_abs * pk.sig_bound));
+    }
+    const bool scanned = affine_suffix_scan<G, COLD>(tid, scr,
         [&](int k, double &mk, double &ck) { mk = U[padp<G>(k)]; ck = X[padp<G>(k)]; },
-        [&](int k, double R) { U[padp<G>(k)] = one_minus_exp_neg(R); });
+        [&](int k, double R) { U[padp<G>(k)] = one_minus_exp_neg(R); },
+        anyc ? __builtin_inf() : 0.0,                      /* a second species: never cold */
+        [&](double sum) { s_line = sum; return sum <= s_lim; });
+    if (!scanned) {
+        rho_max = fmax(rho_max, s_line);
+        __syncthreads();          /* the caller's next exchange overwrites this LDS */
+        return;
+    }
     const int block_anyc = __syncthreads_or(anyc);
     if (block_anyc) {
 #pragma unroll

@@ -42,6 +42,7 @@ struct PlasmaK {
     double deltat;
     double dt_I1e;       /* deltat / I1e: the avalanche rate's factor */
     double rho_nt;
+    double sig_bound;    /* K4 / (2 sqrt K5) + w0 K4 / K5 >= sigma (1 + w0 tau) for every tau */
 };
 
 template <class T> struct TArgs {

@@ -74,13 +74,14 @@ __device__ __forceinline__ void tline_line(const TArgs<T> &a, cx<T> (&v)[G::E], 
         }
         if constexpr (PLASMA) {
             const T half_rho = (T)(0.5 * a.pk.rho_nt), w0 = (T)a.pk.w0;
-            plasma_line<T, G>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
-                               [&](int m, T F, T tau) {
-                                   /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
-                                   const T z = half_rho * F;
-                                   d[m].re -= z;
-                                   d[m].im -= w0 * tau * z;
-                               });
+            plasma_line<T, G, true>(v, tid, reinterpret_cast<double *>(lraw), a.pk, rmax,
+                                     [&](int m, T F, T tau) {
+                                         /* (1 + i w0 tau_c) rho_nt F / 2  (propagation.c:289-292) */
+                                         const T z = half_rho * F;
+                                         d[m].re -= z;
+                                         d[m].im -= w0 * tau * z;
+                                     },
+                                     fabs((double)a.dz));
         }
         /* A <- A exp(dz A_nl)  (propagation.c:317-327, complex product) */
 #pragma unroll
@@ -190,10 +191,11 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
     }
 }
 
-/* the FP32 persistent form is held to two waves per SIMD (256 VGPRs), which
- * it meets with no more than a few spilled registers */
+/* FP32: the persistent form is held to two waves per SIMD (256 VGPRs) and
+ * the per-group form to three (168 VGPRs, which its plasma LDS allows too),
+ * each met with no more than a few spilled registers */
 template <class T, class G, bool PLASMA, bool PERSIST>
-__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), (PERSIST && sizeof(T) == 4) ? 2 : 1)
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), sizeof(T) == 4 ? (PERSIST ? 2 : 3) : 1)
 k_tline(TArgs<T> a) {
     tline_work<T, G, PLASMA, PERSIST>(a, blockIdx.x, gridDim.x, blockIdx.x);
 }
@@ -206,7 +208,7 @@ k_tline(TArgs<T> a) {
  * the same bits as a launch of k_tline<..., false> on it alone.
  */
 template <class T, class G, bool PLASMA>
-__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS))
+__global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), sizeof(T) == 4 ? 3 : 1)
 k_tline_batch(const TArgs<T> *__restrict__ d, const long *__restrict__ gstart, int n) {
     const long b = blockIdx.x;
     int lo = 0, hi = n - 1;

@@ -284,6 +284,46 @@ def test_aperture_and_mirror_loss_match_oracle(gpu_lib, prec, tol):
     assert abs(d["energy_sum"] - ob["energy"]) <= tol * ob["energy"]
 
 
+def _cold_limit(inp, prec):
+    """the largest sum of density-scan constants a line may have and still
+    skip its scans in the t-line (the kernel's thresholds)"""
+    P = inp.params
+    K4 = (P.w0 / 299792458.0) * P.w0 / (P.n0 * P.rho_c)
+    K5 = P.w0 ** 2
+    bound = K4 / (2 * np.sqrt(K5)) + P.w0 * K4 / K5
+    s_max, eps = (2.0 ** -42, 2.0 ** -53) if prec == "f64" else (2.0 ** -24, 2.0 ** -26)
+    return min(s_max, eps / (P.rho_nt * P.dzz * bound))
+
+
+@pytest.mark.parametrize("prec,tol,rtol_rho", [("f64", TOL64, 1e-9), ("f32", TOL32, 1e-4)])
+@pytest.mark.parametrize("energy,all_cold", [(5e-16, True), (1e-10, None), (1e-3, False)])
+def test_cold_lines_match_oracle(gpu_lib, prec, tol, rtol_rho, energy, all_cold):
+    """pulses weak enough that every t-line skips its density scans (its
+    plasma part below the field's rounding, its peak ionised fraction taken as
+    the sum of its scan constants), in between (FP32 cold, FP64 scanned) and
+    intense (hot core, cold wings): field and peak rho against the oracle,
+    which scans every line, over four steps across a mirror"""
+    print(f"case cold lines energy={energy:g} prec={prec}", flush=True)
+    inp, A0 = _case(32, 512, energy=energy, steps_per_pass=3, passes=2)
+    ref, rho_ref = O.run_fixed(inp, A0, 0, 4)
+    frac = rho_ref / inp.params.rho_nt
+    lim = _cold_limit(inp, prec)
+    if all_cold is None:
+        all_cold = prec == "f32"
+    # the case is what it claims, by the oracle's own peak fraction
+    assert (frac < 0.5 * lim) == all_cold
+    assert rho_ref > 0
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, 4)
+        got = p.download()
+        d = p.diagnostics()
+    err = O.rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e} rho gpu={d['rho_peak']:.6e} oracle={rho_ref:.6e} limit={lim:.3e}", flush=True)
+    assert err <= tol
+    assert abs(d["rho_peak"] - rho_ref) <= rtol_rho * rho_ref
+
+
 def test_rho_peak_resets_with_a_new_field(gpu_lib):
     """after a nonlinear run, uploading a field leaves no plasma maximum
     behind: diagnostics read 0 until the next nonlinear step"""
