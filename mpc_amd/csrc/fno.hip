@@ -302,10 +302,17 @@ __global__ void __launch_bounds__(256) k_pointwise(mpc_pw a, const float *__rest
     }
 }
 
+/* outputs per chunk: the chunk's CIN x OC weights sit in SGPRs at once, so
+ * CIN x OC stays within 64 for unit-stride rows (wide scalar loads) and 16
+ * for strided ones (a scalar load each); beyond that the compiler parks
+ * SGPRs in VGPR lanes, two extra instructions per weight */
 template <int CIN, bool UNIT> static void launch_pointwise_u(const mpc_pw *a, dim3 grid, hipStream_t s) {
-    if (a->cout % 8 == 0) hipLaunchKernelGGL((k_pointwise<CIN, 8, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
-    else if (a->cout % 2 == 0)
-        hipLaunchKernelGGL((k_pointwise<CIN, 2, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
+    constexpr int LIM = UNIT ? 64 : 16;
+    constexpr bool C8 = CIN * 8 <= LIM, C2 = CIN * 2 <= LIM;
+    if (C8 && a->cout % 8 == 0)
+        hipLaunchKernelGGL((k_pointwise<CIN, C8 ? 8 : 1, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
+    else if (C2 && a->cout % 2 == 0)
+        hipLaunchKernelGGL((k_pointwise<CIN, C2 ? 2 : 1, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
     else hipLaunchKernelGGL((k_pointwise<CIN, 1, UNIT>), grid, dim3(256), 0, s, *a, a->w, a->bias);
 }
 /* unit input stride (w[o][i] rows): the weight loads merge into wide scalar loads */
