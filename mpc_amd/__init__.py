@@ -238,6 +238,11 @@ class Batch:
     def graphs(self) -> int:
         return lib().mpc_batch_graphs(self.b)
 
+    def autotune(self) -> None:
+        """pick each launch class's t-line form by timing the batch's own
+        launches (after the pulses' own autotune; fields are left as they were)"""
+        check(lib().mpc_batch_autotune(self.b), "mpc_batch_autotune")
+
     def close(self):
         if self.b:
             lib().mpc_batch_destroy(self.b)

@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
         "mpc_batch_destroy": (None, [vp]),
         "mpc_batch_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpc_batch_graphs": (C.c_int, [vp]),
+        "mpc_batch_autotune": (C.c_int, [vp]),
         "mpc_sizeof_gemm": (C.c_int, []),
         "mpc_gemm_bf16": (C.c_int, [vp, vp]),
         "mpc_fno_combine": (C.c_int, [vp, vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),

@@ -763,6 +763,108 @@ template <class T> static int batch_run(mpc_batch *b, int step0, int nsteps, boo
     return 0;
 }
 
+/*
+ * Chooses the t-line form (persistent or one workgroup per line group) of
+ * each launch class (t length, plasma switch) by timing the batch's own
+ * t-line launches: a form that is faster on one pulse alone can be slower
+ * over a group of many.  Runs on scratch copies of the fields (aux1), with
+ * the plasma slots detached, so the pulses' fields and diagnostics are left
+ * as they were.  The x/y variants stay as each context has them.
+ */
+template <class T> static int batch_autotune(mpc_batch *b) {
+    const size_t np = b->ctx.size();
+    std::vector<void *> keep_field(np);
+    std::vector<double *> keep_rho(np);
+    std::vector<int> keep_t(np);
+    for (size_t p = 0; p < np; p++) {
+        mpc_ctx *c = b->ctx[p];
+        const size_t bytes = c->esz * (size_t)c->N;
+        if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
+        HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, b->stream));
+        keep_field[p] = c->field;
+        keep_rho[p] = c->rho_block;
+        keep_t[p] = c->tvar;
+    }
+    for (size_t p = 0; p < np; p++) {
+        b->ctx[p]->field = b->ctx[p]->aux1;
+        b->ctx[p]->rho_block = nullptr;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? 0 : MPC_EHIP;
+    /* the plan of three steps: its second t op (t-spectral in and out, as
+     * every interior step's) is the launch list timed */
+    std::vector<std::vector<SweepOp>> plans(np);
+    size_t ti = 0;
+    for (size_t p = 0; p < np && !rc; p++) {
+        bool fe = false;
+        plan_fixed(b->ctx[p], 0, 3, plans[p], &fe);
+        if (plans[p].size() != plans[0].size()) rc = MPC_EINVAL;
+    }
+    for (int seen = 0; !rc && ti < plans[0].size(); ti++)
+        if (plans[0][ti].kind == 't' && ++seen == 2) break;
+    if (!rc && ti == plans[0].size()) rc = MPC_EINVAL;
+    /* time of a launch: median of 5 samples of k back-to-back launches
+     * (k: enough for about 0.2 ms) after one warm-up launch */
+    auto time_launch = [&](const BLaunch &L) -> float {
+        auto timed = [&](int k) -> float {
+            (void)hipEventRecord(e0, b->stream);
+            for (int i = 0; i < k; i++)
+                if (batch_launch<T>(b, L) != 0) return 1e30f;
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 1e30f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return ms / (float)k;
+        };
+        const float t0 = timed(1);
+        if (t0 >= 1e30f) return 1e30f;
+        const int k = t0 > 0.0f && t0 < 0.2f ? std::min(64, (int)std::ceil(0.2f / t0)) : 1;
+        float t[5];
+        for (int r = 0; r < 5; r++) t[r] = timed(k);
+        std::sort(t, t + 5);
+        return t[2];
+    };
+    /* per class: summed launch time of each form */
+    std::map<std::pair<int, int>, float> best_ms;
+    std::map<std::pair<int, int>, int> best_v;
+    const int nv = tline_variants();
+    for (int v = 0; v < nv && !rc; v++) {
+        for (mpc_ctx *c : b->ctx) c->tvar = v;
+        std::vector<BLaunch> ls;
+        rc = batch_group<T>(b, plans, ti, ls);
+        std::map<std::pair<int, int>, float> sum;
+        for (const BLaunch &L : ls) {
+            if (rc) break;
+            const float ms = time_launch(L);
+            if (ms >= 1e30f) rc = MPC_EHIP;
+            sum[{L.len, L.plasma}] += ms;
+        }
+        for (const auto &kv : sum) {
+            auto it = best_ms.find(kv.first);
+            if (it == best_ms.end() || kv.second < it->second) {
+                best_ms[kv.first] = kv.second;
+                best_v[kv.first] = v;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(b->stream);
+    for (size_t p = 0; p < np; p++) {
+        mpc_ctx *c = b->ctx[p];
+        c->field = keep_field[p];
+        c->rho_block = keep_rho[p];
+        auto it = best_v.find({c->nt, c->p.plasm == 1 ? 1 : 0});
+        c->tvar = (!rc && it != best_v.end()) ? it->second : keep_t[p];
+    }
+    return rc;
+}
+
+extern "C" int mpc_batch_autotune(mpc_batch *b) {
+    if (!b) return MPC_EINVAL;
+    return b->prec == MPC_F32 ? batch_autotune<float>(b) : batch_autotune<double>(b);
+}
+
 extern "C" int mpc_batch_create(mpc_ctx *const *ctxs, int n, mpc_batch **out) {
     if (!ctxs || n <= 0 || !out) return MPC_EINVAL;
     *out = nullptr;
@@ -1250,9 +1352,11 @@ extern "C" int mpc_run_adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem
 /* ------------------------------------------------------------------------ */
 /* FFT / tile autotuner                                                      */
 /* ------------------------------------------------------------------------ */
-/* times every kernel variant (median of 5 launches after a warm-up launch)
+/* times every kernel variant (median of 5 samples after a warm-up launch)
  * on a scratch copy of the resident field and keeps the fastest; the choice
- * only changes speed */
+ * only changes speed.  A sample is enough back-to-back launches to last
+ * about 0.2 ms (at most 64), so that launches of a few microseconds on small
+ * fields are not ranked by timer noise. */
 template <class T> static int autotune(mpc_ctx *c) {
     const size_t bytes = c->esz * (size_t)c->N;
     if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
@@ -1264,18 +1368,26 @@ template <class T> static int autotune(mpc_ctx *c) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    /* one untimed launch (first-use costs), then the median of five */
+    /* one warm-up launch (first-use costs; its time sizes the samples), then
+     * the median of five samples of k launches each */
     auto time_median = [&](auto &&launch) -> float {
-        if (launch() != 0) return 1e30f;
-        float t[5];
-        for (int r = 0; r < 5; r++) {
+        auto timed = [&](int k) -> float {
             (void)hipEventRecord(e0, c->stream);
-            if (launch() != 0) return 1e30f;
+            for (int i = 0; i < k; i++)
+                if (launch() != 0) return 1e30f;
             (void)hipEventRecord(e1, c->stream);
             (void)hipEventSynchronize(e1);
             float ms = 1e30f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            t[r] = ms;
+            return ms / (float)k;
+        };
+        const float t0 = timed(1);
+        if (t0 >= 1e30f) return 1e30f;
+        const int k = t0 > 0.0f && t0 < 0.2f ? std::min(64, (int)std::ceil(0.2f / t0)) : 1;
+        float t[5];
+        for (int r = 0; r < 5; r++) {
+            t[r] = timed(k);
+            if (t[r] >= 1e30f) return 1e30f;
         }
         std::sort(t, t + 5);
         return t[2];

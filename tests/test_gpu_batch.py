@@ -155,3 +155,39 @@ def test_batch_rejects_mixed_schedules(gpu_lib):
     with pytest.raises(RuntimeError):
         Batch([p1, p3])
     _close([p1, p2, p3])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batch_autotune_leaves_fields_and_keeps_results(gpu_lib, prec):
+    """mpc_batch_autotune times the batch's own t-line launches on scratch
+    copies: the fields and the plasma maxima are left as they were, each size
+    class gets one t-line form, and the batched run with the chosen forms is
+    still bit-identical to per-pulse runs with them"""
+    print(f"case batch autotune prec={prec}", flush=True)
+    cases = _inputs()
+    props = _props(cases, prec, (0, 2, 3))
+    for p in props:
+        p.run(0, 2)
+    before = [p.download() for p in props]
+    rho = [p.diagnostics()["rho_peak"] for p in props]
+    with Batch(props) as b:
+        b.autotune()
+        chosen = [p.variants() for p in props]
+        print(f"  t-line forms {[v[0] for v in chosen]}", flush=True)
+        for i, p in enumerate(props):
+            assert np.array_equal(p.download(), before[i])
+            assert p.diagnostics()["rho_peak"] == rho[i]
+            assert chosen[i][1:] == (2, 3)              # x/y variants untouched
+        # one form per (t length, plasma) class: pulses 0 and 3 share one
+        assert chosen[0][0] == chosen[3][0]
+        b.run(2, 4, graph=True)
+        got = [p.download() for p in props]
+    _close(props)
+    for i, (inp, A0) in enumerate(cases):
+        with Propagator(inp, prec) as p:
+            p.upload(A0)
+            p.set_variants(0, 2, 3)
+            p.run(0, 2)
+            p.set_variants(*chosen[i])
+            p.run(2, 4)
+            assert np.array_equal(p.download(), got[i]), (prec, i)
