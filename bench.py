@@ -158,7 +158,7 @@ def against_oracle(inp, A_ref, nsteps, rho_ref):
 
 def _batch_vs_one_by_one(wls, steps, label):
     """pulses advancing `steps` z-steps together (mpc_batch, one launch per
-    sweep and launch group, a HIP graph per pass, the t-line form tuned on the
+    sweep and launch group, a HIP graph per pass, variants tuned on the
     batch's own launches) against the same pulses run one after another with
     their own tuned variants; `steps` must stay below a pass (three calls of
     `steps` steps)"""
@@ -183,8 +183,8 @@ def _batch_vs_one_by_one(wls, steps, label):
         f"grids {sorted(set((w.nx, w.nt) for w in wls))}; variants {tuned}")
     b = Batch(props)
     b.autotune()
-    forms = {key: next(p.variants()[0] for p, w in zip(props, wls) if (w.nx, w.nt) == key) for key in tuned}
-    log(f"[bench] {label}: t-line forms tuned on the batch {forms}")
+    forms = {key: next(p.variants() for p, w in zip(props, wls) if (w.nx, w.nt) == key) for key in tuned}
+    log(f"[bench] {label}: variants tuned on the batch {forms}")
     # the warm-up call has the timed call's launch list (same length, no
     # mirror inside), so the pass graph is captured before the timed region
     b.run(0, steps, graph=True)
@@ -211,7 +211,7 @@ def _batch_vs_one_by_one(wls, steps, label):
             "pulse_steps_per_s_batched": round(len(wls) * steps / dt_b, 1),
             "pulse_steps_per_s_one_by_one": round(len(wls) * steps / dt_s, 1),
             "pulses": len(wls), "steps": steps, "points_total": int(sum(npts)),
-            "tline_forms_batch_tuned": {f"{k[0]}^2x{k[1]}": v for k, v in forms.items()},
+            "variants_batch_tuned": {f"{k[0]}^2x{k[1]}": list(v) for k, v in forms.items()},
             "unit": "256x256x2048-equivalent steps/s", "dtype": "f32"}
 
 

@@ -207,12 +207,14 @@ void mpc_batch_destroy(mpc_batch *b);
 int  mpc_batch_run(mpc_batch *b, int step0, int nsteps, int use_graph);
 int  mpc_batch_graphs(mpc_batch *b);   /* distinct pass graphs captured so far */
 /* Batch-aware tuning (no counterpart in the reference, whose FFTW plans are
- * per size): times the batch's own t-line launches in each form (persistent
- * or one workgroup per line group) and sets every pulse of a launch class
- * (t length, plasma switch) to the faster, since a form faster on one pulse
- * alone can be slower over a group of many.  Runs on scratch copies of the
- * fields; fields and diagnostics are left as they were.  Call after the
- * contexts' own mpc_autotune (it keeps their x/y variants). */
+ * per size): times the batch's own launches and sets every pulse of a launch
+ * class to the fastest variant: the t-line form (persistent or one workgroup
+ * per line group) per t length and plasma switch, and the x/y variants per
+ * line length where a launch covers several pulses (a launch over one pulse
+ * runs the per-field kernel, which the context's own tuning chose).  A
+ * variant faster on one pulse alone can be slower over a group of many.
+ * Runs on scratch copies of the fields; fields and diagnostics are left as
+ * they were.  Call after the contexts' own mpc_autotune. */
 int  mpc_batch_autotune(mpc_batch *b);
 
 /* Timing of the most recent mpc_run: average device time per launch of each

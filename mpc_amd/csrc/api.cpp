@@ -764,20 +764,24 @@ template <class T> static int batch_run(mpc_batch *b, int step0, int nsteps, boo
 }
 
 /*
- * Chooses the t-line form (persistent or one workgroup per line group) of
- * each launch class (t length, plasma switch) by timing the batch's own
- * t-line launches: a form that is faster on one pulse alone can be slower
- * over a group of many.  Runs on scratch copies of the fields (aux1), with
- * the plasma slots detached, so the pulses' fields and diagnostics are left
- * as they were.  The x/y variants stay as each context has them.
+ * Re-tunes the kernel variants on the batch's own launches: the t-line form
+ * (persistent or one workgroup per line group) of each launch class (t
+ * length, plasma switch), and the x/y variants of each class whose launches
+ * cover several pulses (the batched kernels; a launch over one pulse runs the
+ * per-field kernel its context's own tuning chose).  A variant that is
+ * faster on one pulse alone can be slower over a group of many.  Runs on
+ * scratch copies of the fields (aux1), with the plasma slots detached, so the
+ * pulses' fields and diagnostics are left as they were.
  */
 template <class T> static int batch_autotune(mpc_batch *b) {
     const size_t np = b->ctx.size();
     std::vector<void *> keep_field(np);
     std::vector<double *> keep_rho(np);
-    std::vector<int> keep_t(np);
+    std::vector<int> keep_t(np), keep_x(np), keep_y(np);
     for (size_t p = 0; p < np; p++) {
         mpc_ctx *c = b->ctx[p];
+        keep_x[p] = c->avar;
+        keep_y[p] = c->avar_y;
         const size_t bytes = c->esz * (size_t)c->N;
         if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
         HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, b->stream));
@@ -791,18 +795,20 @@ template <class T> static int batch_autotune(mpc_batch *b) {
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? 0 : MPC_EHIP;
-    /* the plan of three steps: its second t op (t-spectral in and out, as
-     * every interior step's) is the launch list timed */
+    /* the plan of three steps: its second t, x and y ops (t-spectral in and
+     * out, as every interior step's) are the launch lists timed */
     std::vector<std::vector<SweepOp>> plans(np);
-    size_t ti = 0;
     for (size_t p = 0; p < np && !rc; p++) {
         bool fe = false;
         plan_fixed(b->ctx[p], 0, 3, plans[p], &fe);
         if (plans[p].size() != plans[0].size()) rc = MPC_EINVAL;
     }
-    for (int seen = 0; !rc && ti < plans[0].size(); ti++)
-        if (plans[0][ti].kind == 't' && ++seen == 2) break;
-    if (!rc && ti == plans[0].size()) rc = MPC_EINVAL;
+    auto second_op = [&](char kind) -> size_t {
+        int seen = 0;
+        for (size_t i = 0; i < plans[0].size(); i++)
+            if (plans[0][i].kind == kind && ++seen == 2) return i;
+        return plans[0].size();
+    };
     /* time of a launch: median of 5 samples of k back-to-back launches
      * (k: enough for about 0.2 ms) after one warm-up launch */
     auto time_launch = [&](const BLaunch &L) -> float {
@@ -824,29 +830,47 @@ template <class T> static int batch_autotune(mpc_batch *b) {
         std::sort(t, t + 5);
         return t[2];
     };
-    /* per class: summed launch time of each form */
-    std::map<std::pair<int, int>, float> best_ms;
-    std::map<std::pair<int, int>, int> best_v;
-    const int nv = tline_variants();
-    for (int v = 0; v < nv && !rc; v++) {
-        for (mpc_ctx *c : b->ctx) c->tvar = v;
-        std::vector<BLaunch> ls;
-        rc = batch_group<T>(b, plans, ti, ls);
-        std::map<std::pair<int, int>, float> sum;
-        for (const BLaunch &L : ls) {
-            if (rc) break;
-            const float ms = time_launch(L);
-            if (ms >= 1e30f) rc = MPC_EHIP;
-            sum[{L.len, L.plasma}] += ms;
-        }
-        for (const auto &kv : sum) {
-            auto it = best_ms.find(kv.first);
-            if (it == best_ms.end() || kv.second < it->second) {
-                best_ms[kv.first] = kv.second;
-                best_v[kv.first] = v;
+    /* Tunes one sweep kind: every variant is set on every pulse (set_v), the
+     * kind's launches of the op are built and timed, and each launch class
+     * (line length, plasma switch) keeps its fastest variant in best_v.  With
+     * multi_only, only launches over several pulses count (a launch over one
+     * pulse runs the per-field kernel its context's own tuning chose), and a
+     * class with none of them is left out. */
+    using Key = std::pair<int, int>;
+    auto tune_kind = [&](char kind, int nvar, bool multi_only, auto &&set_v, std::map<Key, int> &best_v) {
+        const size_t oi = second_op(kind);
+        if (rc || oi == plans[0].size()) return;
+        (void)hipGetLastError();
+        std::map<Key, float> best_ms;
+        for (int v = 0; v < nvar && !rc; v++) {
+            for (mpc_ctx *c : b->ctx) set_v(c, v);
+            std::vector<BLaunch> ls;
+            if (batch_group<T>(b, plans, oi, ls) != 0) continue;     /* a variant that does not fit: skipped */
+            std::map<Key, float> sum;
+            bool ok = true;
+            for (const BLaunch &L : ls) {
+                if (multi_only && !L.single.empty()) continue;
+                const float ms = time_launch(L);
+                if (ms >= 1e30f) { ok = false; break; }
+                sum[{L.len, L.plasma}] += ms;
+            }
+            if (!ok) continue;
+            for (const auto &kv : sum) {
+                auto it = best_ms.find(kv.first);
+                if (it == best_ms.end() || kv.second < it->second) {
+                    best_ms[kv.first] = kv.second;
+                    best_v[kv.first] = v;
+                }
             }
         }
-    }
+    };
+    std::map<Key, int> bt, bx, by;
+    tune_kind('t', tline_variants(), false, [](mpc_ctx *c, int v) { c->tvar = v; }, bt);
+    for (size_t p = 0; p < np; p++) b->ctx[p]->tvar = keep_t[p];
+    tune_kind('x', axis_variants(), true, [](mpc_ctx *c, int v) { c->avar = v; }, bx);
+    for (size_t p = 0; p < np; p++) b->ctx[p]->avar = keep_x[p];
+    tune_kind('y', axis_variants(), true, [](mpc_ctx *c, int v) { c->avar_y = v; }, by);
+    for (size_t p = 0; p < np; p++) b->ctx[p]->avar_y = keep_y[p];
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(b->stream);
@@ -854,8 +878,13 @@ template <class T> static int batch_autotune(mpc_batch *b) {
         mpc_ctx *c = b->ctx[p];
         c->field = keep_field[p];
         c->rho_block = keep_rho[p];
-        auto it = best_v.find({c->nt, c->p.plasm == 1 ? 1 : 0});
-        c->tvar = (!rc && it != best_v.end()) ? it->second : keep_t[p];
+        if (rc) continue;
+        auto it = bt.find({c->nt, c->p.plasm == 1 ? 1 : 0});
+        if (it != bt.end()) c->tvar = it->second;
+        auto ix = bx.find({c->nx, 0});
+        if (ix != bx.end()) c->avar = ix->second;
+        auto iy = by.find({c->ny, 0});
+        if (iy != by.end()) c->avar_y = iy->second;
     }
     return rc;
 }

@@ -159,10 +159,10 @@ def test_batch_rejects_mixed_schedules(gpu_lib):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_batch_autotune_leaves_fields_and_keeps_results(gpu_lib, prec):
-    """mpc_batch_autotune times the batch's own t-line launches on scratch
-    copies: the fields and the plasma maxima are left as they were, each size
-    class gets one t-line form, and the batched run with the chosen forms is
-    still bit-identical to per-pulse runs with them"""
+    """mpc_batch_autotune times the batch's own launches on scratch copies:
+    the fields and the plasma maxima are left as they were, each launch class
+    gets one variant, and the batched run with the chosen variants is still
+    bit-identical to per-pulse runs with them"""
     print(f"case batch autotune prec={prec}", flush=True)
     cases = _inputs()
     props = _props(cases, prec, (0, 2, 3))
@@ -173,13 +173,16 @@ def test_batch_autotune_leaves_fields_and_keeps_results(gpu_lib, prec):
     with Batch(props) as b:
         b.autotune()
         chosen = [p.variants() for p in props]
-        print(f"  t-line forms {[v[0] for v in chosen]}", flush=True)
+        print(f"  variants chosen {chosen}", flush=True)
         for i, p in enumerate(props):
             assert np.array_equal(p.download(), before[i])
             assert p.diagnostics()["rho_peak"] == rho[i]
-            assert chosen[i][1:] == (2, 3)              # x/y variants untouched
-        # one form per (t length, plasma) class: pulses 0 and 3 share one
+        # one t-line form per (t length, plasma) class: pulses 0 and 3 share
+        # one; pulses 0, 2 and 3 (32-point lines) share x and y variants;
+        # pulse 1 is alone in its x/y class and keeps its own
         assert chosen[0][0] == chosen[3][0]
+        assert chosen[0][1:] == chosen[2][1:] == chosen[3][1:]
+        assert chosen[1][1:] == (2, 3)
         b.run(2, 4, graph=True)
         got = [p.download() for p in props]
     _close(props)
