@@ -793,6 +793,18 @@ template <class T> static int batch_autotune(mpc_batch *b) {
         b->ctx[p]->field = b->ctx[p]->aux1;
         b->ctx[p]->rho_block = nullptr;
     }
+    /* the scratch copies in the state a run keeps between steps (t-spectral,
+     * unnormalised), so the timed t-lines see the run's intensities */
+    (void)hipStreamSynchronize(b->stream);
+    for (size_t p = 0; p < np; p++) {
+        mpc_ctx *c = b->ctx[p];
+        if (!c->domain_freq && sweep_t<T>(c, nullptr, nullptr, false, false, false, 0.0) != 0) {
+            c->field = keep_field[p];
+            c->rho_block = keep_rho[p];
+            return MPC_EHIP;
+        }
+    }
+    for (mpc_ctx *c : b->ctx) (void)hipStreamSynchronize(c->stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? 0 : MPC_EHIP;
     /* the plan of three steps: its second t, x and y ops (t-spectral in and
@@ -809,6 +821,18 @@ template <class T> static int batch_autotune(mpc_batch *b) {
             if (plans[0][i].kind == kind && ++seen == 2) return i;
         return plans[0].size();
     };
+    /* The timed t op applies the dispersion table itself (which carries the
+     * 1/Nt of the transform pair), as a run's first step does: repeated on
+     * the scratch copy it then keeps the state a run keeps between steps.
+     * (In a run the y sweep before it applies the table instead.) */
+    {
+        const size_t oi = second_op('t');
+        if (oi < plans[0].size())
+            for (size_t p = 0; p < np; p++) {
+                plans[p][oi].has_p = true;
+                plans[p][oi].pcoef = b->ctx[p]->p.dzz;
+            }
+    }
     /* time of a launch: median of 5 samples of k back-to-back launches
      * (k: enough for about 0.2 ms) after one warm-up launch */
     auto time_launch = [&](const BLaunch &L) -> float {
@@ -1385,18 +1409,25 @@ extern "C" int mpc_run_adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem
  * on a scratch copy of the resident field and keeps the fastest; the choice
  * only changes speed.  A sample is enough back-to-back launches to last
  * about 0.2 ms (at most 64), so that launches of a few microseconds on small
- * fields are not ranked by timer noise. */
+ * fields are not ranked by timer noise; the t-line forms' samples
+ * interleave. */
 template <class T> static int autotune(mpc_ctx *c) {
     const size_t bytes = c->esz * (size_t)c->N;
     if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
     HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, c->stream));
     void *keep = c->field;
     const int keep_t = c->tvar, keep_a = c->avar, keep_y = c->avar_y, keep_timing = c->timing;
-    c->field = c->aux1;
-    c->timing = 0;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return MPC_EHIP;
+    }
+    c->field = c->aux1;
+    c->timing = 0;
+    /* the scratch copy in the state a run keeps between steps (t-spectral,
+     * unnormalised), so the timed t-lines see the run's intensities */
+    int rc0 = c->domain_freq ? 0 : sweep_t<T>(c, nullptr, nullptr, false, false, false, 0.0);
     /* one warm-up launch (first-use costs; its time sizes the samples), then
      * the median of five samples of k launches each */
     auto time_median = [&](auto &&launch) -> float {
@@ -1424,7 +1455,7 @@ template <class T> static int autotune(mpc_ctx *c) {
     const double dz = c->p.dzz;
     const cx<T> *D = get_table<T>(c, 'x', dz);
     const cx<T> *P = get_table<T>(c, 't', dz);
-    int rc = (!D || !P) ? MPC_ENOMEM : 0;
+    int rc = rc0 ? rc0 : (!D || !P) ? MPC_ENOMEM : 0;
     float bestX = 1e30f, bestY = 1e30f, bestT = 1e30f;
     int vx = keep_a, vy = keep_y, vt = keep_t;
     for (int v = 0; !rc && v < axis_variants(); v++) {
@@ -1434,10 +1465,48 @@ template <class T> static int autotune(mpc_ctx *c) {
         if (mx < bestX) { bestX = mx; vx = v; }
         if (my < bestY) { bestY = my; vy = v; }
     }
-    for (int v = 0; !rc && v < tline_variants(); v++) {
-        c->tvar = v;
-        const float ms = time_median([&] { return sweep_t<T>(c, P, nullptr, true, true, false, dz); });
-        if (ms < bestT) { bestT = ms; vt = v; }
+    /* The t-line forms are timed over whole steps (x sweep, y sweep, t-line,
+     * with the x/y variants just chosen), since what the sweeps before it
+     * leave in the caches differs from back-to-back t-lines; their samples
+     * interleave (each step advances the scratch field's nonlinear state,
+     * which the plasma work depends on). */
+    {
+        const int nvt = tline_variants();
+        std::vector<std::vector<float>> ts(nvt);
+        c->avar = vx;
+        c->avar_y = vy;
+        auto launch_t = [&] {
+            int r = sweep_axis<T>(c, 0, D, false);
+            if (!r) r = sweep_axis<T>(c, 1, D, false);
+            if (!r) r = sweep_t<T>(c, P, nullptr, true, true, false, dz);
+            return r;
+        };
+        auto sample = [&](int k) -> float {
+            (void)hipEventRecord(e0, c->stream);
+            for (int i = 0; i < k; i++)
+                if (launch_t() != 0) return 1e30f;
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 1e30f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return ms / (float)k;
+        };
+        float tw = 1e30f;                                        /* warm-up; sizes the samples */
+        for (int v = 0; !rc && v < nvt; v++) {
+            c->tvar = v;
+            tw = std::min(tw, sample(1));
+        }
+        const int k = tw > 0.0f && tw < 0.2f ? std::min(64, (int)std::ceil(0.2f / tw)) : 1;
+        for (int r = 0; !rc && r < 5; r++)
+            for (int v = 0; v < nvt; v++) {
+                c->tvar = v;
+                ts[v].push_back(sample(k));
+            }
+        for (int v = 0; !rc && v < nvt; v++) {
+            std::sort(ts[v].begin(), ts[v].end());
+            const float ms = ts[v][ts[v].size() / 2];
+            if (ms < bestT) { bestT = ms; vt = v; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
