@@ -796,17 +796,14 @@ template <class T> static int batch_autotune(mpc_batch *b) {
     /* the scratch copies in the state a run keeps between steps (t-spectral,
      * unnormalised), so the timed t-lines see the run's intensities */
     (void)hipStreamSynchronize(b->stream);
-    for (size_t p = 0; p < np; p++) {
+    int rc = 0;
+    for (size_t p = 0; p < np && !rc; p++) {
         mpc_ctx *c = b->ctx[p];
-        if (!c->domain_freq && sweep_t<T>(c, nullptr, nullptr, false, false, false, 0.0) != 0) {
-            c->field = keep_field[p];
-            c->rho_block = keep_rho[p];
-            return MPC_EHIP;
-        }
+        if (!c->domain_freq) rc = sweep_t<T>(c, nullptr, nullptr, false, false, false, 0.0);
     }
     for (mpc_ctx *c : b->ctx) (void)hipStreamSynchronize(c->stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? 0 : MPC_EHIP;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MPC_EHIP;
     /* the plan of three steps: its second t, x and y ops (t-spectral in and
      * out, as every interior step's) are the launch lists timed */
     std::vector<std::vector<SweepOp>> plans(np);
