@@ -659,7 +659,7 @@ __device__ __forceinline__ void opaque_chain(TwChain<T, N, E, Ns, R, Rest...> &c
 #pragma unroll
         for (int b = 0; b < E / R; ++b)
 #pragma unroll
-            for (int i = 0; i < tw_npow<R>(); ++i) asm volatile("" : "+v"(ch.p[b][i].re), "+v"(ch.p[b][i].im));
+            for (int i = 0; i < tw_npow<R, T>(); ++i) asm volatile("" : "+v"(ch.p[b][i].re), "+v"(ch.p[b][i].im));
     }
     opaque_chain(ch.next);
 }

@@ -199,19 +199,29 @@ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
 /*
  * Stage twiddles w[r] = W^(r*idx) (conjugated for the inverse), r < R, from
- * the table W^m = tw[m] = exp(-2 pi i m / N).  Only the power-of-two powers
- * are loaded (load_powers); the rest are products of loaded entries, at most
- * three multiplications deep (expand_powers).  That cuts the table loads of a
- * radix-8 stage from 7 to 3 and of a radix-16 stage from 15 to 4.  r*idx < N,
- * so no wrap.  The split lets a kernel load the powers once and expand them
- * for every transform it runs.
+ * the table W^m = tw[m] = exp(-2 pi i m / N), each entry rounded once from
+ * its exact value.  r*idx < N, so no wrap.  The split (load_powers, then
+ * expand_powers) lets a kernel load them once and use them for every
+ * transform it runs.
+ * FP32 loads every twiddle.  A product of rounded entries carries their
+ * rounding errors and its own, the same in every transform, and over hundreds
+ * of steps of FP32 transform pairs that drift grew linearly, at 2-4x the
+ * drift with every twiddle loaded.
+ * FP64 loads only the power-of-two powers and forms the rest as products, at
+ * most three multiplications deep: a radix-8 stage loads 3 entries instead of
+ * 7, a radix-16 stage 4 instead of 15, and FP64 products stay far below what
+ * an FP64 run resolves.
  */
-template <int R> constexpr int tw_npow() {
-    return R == 2 ? 1 : (R == 3 || R == 4) ? 2 : R == 8 ? 3 : R == 16 ? 4 : R - 1;
+template <class T> constexpr bool tw_products(int R) {
+    return sizeof(T) == 8 && (R == 2 || R == 3 || R == 4 || R == 8 || R == 16);
+}
+template <int R, class T> constexpr int tw_npow() {
+    if constexpr (tw_products<T>(R)) return R == 2 ? 1 : (R == 3 || R == 4) ? 2 : R == 8 ? 3 : 4;
+    else return R - 1;
 }
 template <int R, class T>
-MPC_HD void load_powers(cx<T> (&p)[tw_npow<R>()], const cx<T> *tw, int idx) {
-    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) {
+MPC_HD void load_powers(cx<T> (&p)[(tw_npow<R, T>())], const cx<T> *tw, int idx) {
+    if constexpr (tw_products<T>(R)) {
         p[0] = tw[idx];
         if constexpr (R > 2) p[1] = tw[2 * idx];
         if constexpr (R >= 8) p[2] = tw[4 * idx];
@@ -222,13 +232,13 @@ MPC_HD void load_powers(cx<T> (&p)[tw_npow<R>()], const cx<T> *tw, int idx) {
     }
 }
 template <int R, int DIR, class T>
-MPC_HD void expand_powers(cx<T> (&w)[R], const cx<T> (&p)[tw_npow<R>()]) {
+MPC_HD void expand_powers(cx<T> (&w)[R], const cx<T> (&p)[(tw_npow<R, T>())]) {
     auto cj = [](cx<T> v) {
         if (DIR > 0) v.im = -v.im;
         return v;
     };
     w[0] = mk((T)1, (T)0);
-    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) {
+    if constexpr (tw_products<T>(R)) {
         w[1] = cj(p[0]);
         if constexpr (R > 2) {
             w[2] = cj(p[1]);
@@ -254,8 +264,8 @@ MPC_HD void expand_powers(cx<T> (&w)[R], const cx<T> (&p)[tw_npow<R>()]) {
 }
 template <int R, int DIR, class T>
 MPC_HD void stage_twiddles(cx<T> (&w)[R], const cx<T> *tw, int idx) {
-    cx<T> p[tw_npow<R>()];
-    load_powers<R>(p, tw, idx);
+    cx<T> p[(tw_npow<R, T>())];
+    load_powers<R, T>(p, tw, idx);
     expand_powers<R, DIR>(w, p);
 }
 
@@ -266,7 +276,7 @@ template <class T, int N, int R, int Ns, int DIR> struct TwFromTable {
     MPC_HD void operator()(int, int k, cx<T> (&w)[R]) const { stage_twiddles<R, DIR>(w, tw, k * (N / (Ns * R))); }
 };
 template <class T, int R, int B, int DIR> struct TwFromPowers {
-    const cx<T> (*p)[tw_npow<R>()];   /* p[b] */
+    const cx<T> (*p)[(tw_npow<R, T>())];   /* p[b] */
     MPC_HD void operator()(int b, int, cx<T> (&w)[R]) const { expand_powers<R, DIR>(w, p[b]); }
 };
 
@@ -319,13 +329,13 @@ template <class T, int N, int E, int Ns, int... Rs> struct TwChain {
     MPC_HD void load(const cx<T> *, int) {}
 };
 template <class T, int N, int E, int Ns, int R, int... Rest> struct TwChain<T, N, E, Ns, R, Rest...> {
-    static constexpr int B = E / R, NP = tw_npow<R>(), TPL = N / E;
+    static constexpr int B = E / R, NP = tw_npow<R, T>(), TPL = N / E;
     cx<T> p[B][NP];
     TwChain<T, N, E, Ns * R, Rest...> next;
     MPC_HD void load(const cx<T> *tw, int tid) {
         if constexpr (Ns > 1) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) load_powers<R>(p[b], tw, ((tid + TPL * b) % Ns) * (N / (Ns * R)));
+            for (int b = 0; b < B; ++b) load_powers<R, T>(p[b], tw, ((tid + TPL * b) % Ns) * (N / (Ns * R)));
         }
         next.load(tw, tid);
     }
