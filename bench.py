@@ -293,6 +293,37 @@ def adaptive_bench(steps, warmup):
                        "per_precision": out}}
 
 
+def drift_bench(every=100):
+    """FP32 against FP64 over the first pass of the 256x256x2048 workload (500
+    steps and the mirror): rel L2 of the FP32 field against the FP64 field
+    every `every` steps, both from the same input with the radix-16 x/y plans
+    the tuner picks at this size (the FP64 path is the one the oracle pins)"""
+    import numpy as np
+    from mpc_amd import Propagator, configs, make_inputs
+    wl = configs.cfg2_production_256()
+    inp = make_inputs(wl)
+    A0 = inp.calibrated()
+    fields = {}
+    for prec, var in (("f64", (1, 16, 28)), ("f32", (1, 16, 24))):
+        with Propagator(inp, prec) as p:
+            p.upload(A0)
+            p.set_variants(*var)
+            for k in range(0, wl.steps_per_pass, every):
+                p.run(k, min(every, wl.steps_per_pass - k))
+                fields[(prec, k + every)] = p.download()
+        log(f"[bench] drift: {prec} pass done")
+    out = {}
+    for n in range(every, wl.steps_per_pass + 1, every):
+        a, b = fields[("f32", n)], fields[("f64", n)]
+        out[n] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        log(f"[bench] drift: after {n} steps fp32 vs fp64 rel L2 {out[n]:.3e}")
+    return {"metric": "FP32 vs FP64 field rel L2 over the first pass at 256x256x2048",
+            "value": out[wl.steps_per_pass], "unit": "rel L2", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "f32 vs f64", "data": "synthetic (the 256x256x2048 input)", "vs_baseline": None,
+            "config": {"workload": "256x256x2048, 500 steps per pass, mirror after step 499",
+                       "rel_l2_by_steps": out}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +344,8 @@ def main(argv=None):
     ap.add_argument("--train-seconds", type=float, default=240.0, help=argparse.SUPPRESS)
     ap.add_argument("--surrogate-width", type=int, default=32, help=argparse.SUPPRESS)
     ap.add_argument("--surrogate-layers", type=int, default=4, help=argparse.SUPPRESS)
+    ap.add_argument("--drift", action="store_true",
+                    help="instead: FP32 against FP64 every 100 steps over the first 500-step pass")
     ap.add_argument("--prec", choices=["both", "f32", "f64"], default="both",
                     help="f32 / f64 alone skips the other path and the fp32-vs-fp64 comparison (profiling runs)")
     ap.add_argument("--nt", type=int, default=2048, help=argparse.SUPPRESS)
@@ -332,6 +365,9 @@ def main(argv=None):
         return 0
     if a.adaptive:
         print(json.dumps(adaptive_bench(a.steps, a.warmup)))
+        return 0
+    if a.drift:
+        print(json.dumps(drift_bench()))
         return 0
     if a.surrogate:
         from mpc_amd import surrogate
