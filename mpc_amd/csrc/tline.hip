@@ -192,8 +192,10 @@ __device__ __forceinline__ void tline_work(const TArgs<T> &a, long grp, long gst
 }
 
 /* FP32: the persistent form is held to two waves per SIMD (256 VGPRs) and
- * the per-group form to three (168 VGPRs, which its plasma LDS allows too),
- * each met with no more than a few spilled registers */
+ * the per-group form to three (168 VGPRs, which its plasma LDS allows too).
+ * The persistent form fits; the per-group form spills (tens of VGPRs at 1024
+ * and 2048 points, since it loads every stage twiddle), and the tuner keeps
+ * whichever is faster */
 template <class T, class G, bool PLASMA, bool PERSIST>
 __global__ void __launch_bounds__((TCfg<T, G, PLASMA>::THREADS), sizeof(T) == 4 ? (PERSIST ? 2 : 3) : 1)
 k_tline(TArgs<T> a) {
