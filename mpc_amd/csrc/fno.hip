@@ -6,13 +6,13 @@
  * truncated DFTs along t, y and x, their inverses and transposes, and the
  * weight gradients of the 1x1 channel maps, as batched GEMMs (k_gemm_bf16):
  * operands are rounded to BF16 as they are staged into LDS, products
- * accumulate in FP32 (v_mfma_f32_16x16x16_bf16).  On the vector units, in
- * FP32: the pointwise 1x1 channel maps with bias, residual and GELU fused
- * (k_pointwise; at <= 64 channels a point is a few hundred FMAs against its
- * traffic, so a matrix-core tiling would only add staging), the per-mode
- * complex channel mixing and its backward (k_mix, k_mix_bwd), complex
- * recombination after the real-block DFT products, bias reductions, the loss
- * and the Adam update.
+ * accumulate in FP32 (v_mfma_f32_16x16x16_bf16).  The pointwise 1x1 channel
+ * maps with bias, residual and GELU fused: FP32 on the matrix cores
+ * (k_pointwise_mf, v_mfma_f32_32x32x2f32) at 16 to 64 input channels, on the
+ * vector units otherwise (k_pointwise).  On the vector units, in FP32: the
+ * per-mode complex channel mixing and its backward (k_mix, k_mix_bwd),
+ * complex recombination after the real-block DFT products, bias reductions,
+ * the loss and the Adam update.
  */
 #include "internal.h"
 #include "../../include/mpc_fno.h"
@@ -239,10 +239,9 @@ __global__ void k_act_bwd(const float *__restrict__ z, const float *__restrict__
  * Pointwise channel map (a 1x1 convolution) with its epilogue fused:
  *   z[b][o][p] = sum_i w[o*ldo + i*ldi] x[b][i][p] + beta zin[b][o][p] + bias[o]
  *   out[b][o][p] = GELU(z) (act) or z;  zs, if given, receives z.
- * FP32 on the vector ALUs: at <= 64 channels a point costs cin*cout FMAs
- * against 4 (cin + cout (+ cout)) bytes of HBM traffic, so the map is bound
- * by HBM, and a matrix-core tiling would only add staging.  One thread per
- * point (coalesced along p); the weights are wave-uniform (scalar loads).
+ * FP32 on the vector ALUs, for the widths the matrix-core form below does
+ * not take (cin < 16 or cout not a multiple of 32).  One thread per point
+ * (coalesced along p); the weights are wave-uniform (scalar loads).
  * out may alias zin or zs (each element is read before it is written, by the
  * same thread).
  */
@@ -319,6 +318,116 @@ template <int CIN, bool UNIT> static void launch_pointwise_u(const mpc_pw *a, di
 template <int CIN> static void launch_pointwise(const mpc_pw *a, dim3 grid, hipStream_t s) {
     if (a->ldi == 1) launch_pointwise_u<CIN, true>(a, grid, s);
     else launch_pointwise_u<CIN, false>(a, grid, s);
+}
+
+/*
+ * The same map on the FP32 matrix cores, for cin in {16, 32, 64} and cout a
+ * multiple of 32.  At these widths the vector ALUs spend cin FMAs per output
+ * and run behind HBM; here a wave forms a tile of 32 outputs x 32 points as
+ * cin/2 products v_mfma_f32_32x32x2f32 (K = 2 each), and the vector ALUs
+ * are left the epilogue (residual, bias, GELU).
+ * Operand layout of v_mfma_f32_32x32x2f32 (lane l, 64 lanes):
+ *   A (32 x 2): lane l supplies A[l % 32][l / 32]  -> w[o = l % 32][i = l / 32]
+ *   B (2 x 32): lane l supplies B[l / 32][l % 32]  -> x[i = l / 32][p = l % 32]
+ *   D (32 x 32): register r of lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+ * so each x load reads two channel rows of 32 consecutive points, and each
+ * epilogue register stores two output rows of 32 consecutive points.
+ * A wave keeps its OT x 32 outputs' weights in registers and walks point tiles
+ * grid-stride; blockIdx.z selects the group of OT x 32 outputs, blockIdx.y
+ * the sample.  The tile loop is uniform per wave (MFMA runs on all 64 lanes):
+ * the lanes past the last point load the last point and skip their stores.
+ * out may alias zin or zs: each element is read and written by one lane.
+ */
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int PW_MF_WAVES = 4;
+
+template <int CIN, int OT>
+__global__ void __launch_bounds__(64 * PW_MF_WAVES) k_pointwise_mf(mpc_pw a, const float *__restrict__ w,
+                                                                   const float *__restrict__ bias) {
+    constexpr int KS = CIN / 2;
+    const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
+    const int b = blockIdx.y, o_base = blockIdx.z * (OT * 32);
+    float av[OT][KS], bv[OT][16];
+#pragma unroll
+    for (int t = 0; t < OT; ++t) {
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            av[t][s] = w[(long)(o_base + t * 32 + col) * a.ldo + (long)(2 * s + half) * a.ldi];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) bv[t][r] = bias ? bias[o_base + t * 32 + 8 * (r / 4) + 4 * half + r % 4] : 0.f;
+    }
+    /* addresses: a wave-uniform row pointer plus a 32-bit lane offset (the
+     * half-wave's row and the point), so each load and store takes one VGPR
+     * for its address */
+    const float *xb = a.x + b * a.x_bs;
+    const float *zb = a.beta != 0.f ? a.zin + b * a.zin_bs : nullptr;
+    float *ob = a.out + (long)b * a.cout * a.np;
+    float *sb = a.zs ? a.zs + (long)b * a.cout * a.np : nullptr;
+    const long ntile = (a.np + 31) / 32;
+    for (long tile = blockIdx.x * (long)PW_MF_WAVES + (threadIdx.x >> 6); tile < ntile;
+         tile += (long)gridDim.x * PW_MF_WAVES) {
+        const long p0 = tile * 32 + col;
+        const bool live = p0 < a.np;
+        const unsigned p = (unsigned)(live ? p0 : a.np - 1);
+        const unsigned xo = (unsigned)(half * a.np) + p, zo = (unsigned)(4 * half * a.np) + p;
+        float xv[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) xv[s] = (xb + 2 * s * a.np)[xo];
+        /* the tile's residual loads go out with its x loads, ahead of every
+         * store (out may alias zin, so the compiler keeps loads behind
+         * earlier stores as written) */
+        float zv[OT][16];
+#pragma unroll
+        for (int t = 0; t < OT; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                zv[t][r] = zb ? (zb + (long)(o_base + t * 32 + 8 * (r / 4) + r % 4) * a.np)[zo] : 0.f;
+#pragma unroll
+        for (int t = 0; t < OT; ++t) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t][s], xv[s], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long row = (long)(o_base + t * 32 + 8 * (r / 4) + r % 4) * a.np;
+                float z = acc[r] + a.beta * zv[t][r];
+                if (bias) z += bv[t][r];
+                if (live) {
+                    if (sb) (sb + row)[zo] = z;
+                    (ob + row)[zo] = a.act ? gelu(z) : z;
+                }
+            }
+        }
+    }
+}
+
+static bool pointwise_mf_fits(const mpc_pw *a) {
+    /* the lane offsets (below 5 np) are 32-bit */
+    return (a->cin == 16 || a->cin == 32 || a->cin == 64) && a->cout % 32 == 0 && a->cout / 32 <= 65535 &&
+           a->np <= (1L << 32) / 5;
+}
+/* workgroups of k resident on the current device at once (0: unknown) */
+static long pw_resident(const void *k) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 64 * PW_MF_WAVES, 0) != hipSuccess)
+        return 0;
+    return (long)ncu * per;
+}
+template <int CIN, int OT> static void launch_pointwise_mf_t(const mpc_pw *a, hipStream_t s) {
+    static const long resident = pw_resident((const void *)k_pointwise_mf<CIN, OT>);
+    const long ntile = (a->np + 31) / 32, gz = a->cout / (32 * OT);
+    /* one resident wave set walks the tiles: no second, partly filled round
+     * of workgroups, and the weights load once per wave */
+    long gx = (ntile + PW_MF_WAVES - 1) / PW_MF_WAVES;
+    if (resident > 0) gx = std::min(gx, std::max(1L, resident / (a->nb * gz)));
+    hipLaunchKernelGGL((k_pointwise_mf<CIN, OT>), dim3((unsigned)gx, a->nb, (unsigned)gz), dim3(64 * PW_MF_WAVES), 0,
+                       s, *a, a->w, a->bias);
+}
+template <int CIN> static void launch_pointwise_mf(const mpc_pw *a, hipStream_t s) {
+    if (a->cout % 64 == 0) launch_pointwise_mf_t<CIN, 2>(a, s);
+    else launch_pointwise_mf_t<CIN, 1>(a, s);
 }
 
 /*
@@ -562,6 +671,14 @@ extern "C" int mpc_fno_pointwise(const mpc_pw *a, void *stream) {
         return MPC_EINVAL;
     const dim3 grid((unsigned)((a->np + 255) / 256), (unsigned)a->nb);
     hipStream_t s = (hipStream_t)stream;
+    if (pointwise_mf_fits(a)) {
+        switch (a->cin) {
+        case 16: launch_pointwise_mf<16>(a, s); break;
+        case 32: launch_pointwise_mf<32>(a, s); break;
+        default: launch_pointwise_mf<64>(a, s); break;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+    }
     switch (a->cin) {
     case 1: launch_pointwise<1>(a, grid, s); break;
     case 2: launch_pointwise<2>(a, grid, s); break;
