@@ -36,7 +36,9 @@ def _t(a):
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 32, False, False), (70, 33, 45, True, False),
                                          (17, 130, 64, False, True), (100, 90, 200, True, True),
-                                         (32, 32, 8192, False, True)])
+                                         (32, 32, 8192, False, True),
+                                         # ragged M and N, aligned unit-stride rows
+                                         (40, 96, 100, False, False)])
 def test_gemm_matches_bf16_product(gpu_lib, M, N, K, ta, tb):
     from mpc_amd.fno import Ops
     print(f"case gemm M={M} N={N} K={K} ta={ta} tb={tb}", flush=True)
