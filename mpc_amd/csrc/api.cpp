@@ -254,12 +254,14 @@ template <class T> static const cx<T> *get_table(mpc_ctx *c, char kind, double c
     auto it = c->tables.find(key);
     if (it != c->tables.end()) return (const cx<T> *)it->second;
     std::vector<cx<T>> tab;
-    if (kind == 't') {
+    if (kind == 't' || kind == 'n') {
+        /* 't': carries the 1/nt of one forward/inverse t-FFT pair; 'n': the
+         * same factor without it */
         const int nt = c->nt;
         tab.resize(nt);
         for (int k = 0; k < nt; k++) {
             const double beta = c->betas[(k + nt / 2) % nt];
-            const double mag = std::exp(-c->p.alpha / 2.0 * coef) / nt;
+            const double mag = std::exp(-c->p.alpha / 2.0 * coef) / (kind == 't' ? nt : 1);
             const double ph = reduce_2pi(beta * coef);
             tab[k] = mk((T)(mag * std::cos(ph)), (T)(mag * std::sin(ph)));
         }
@@ -1036,10 +1038,6 @@ template <class T> static int step_once(mpc_ctx *c, double dzz, double z, int *s
     return run_xy<T>(c, s * dzz / 2, false);
 }
 
-/* two consecutive isolated steps of dzz at z and z + dzz, with the middle
- * half steps merged (their linear factors commute): x y | t | x y (+P on the
- * t-spectral field) | t | x y, 8 sweeps instead of 10.  signmem is updated
- * as two step_once calls would. */
 /* the fused end of an adaptive attempt: C1's buffer and the partial sums
  * the last y sweep of C2 writes (launch_axis_richardson) */
 struct RichOut {
@@ -1047,8 +1045,36 @@ struct RichOut {
     double *err_part = nullptr, *lp_part = nullptr;
 };
 
-template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *signmem, const void *src = nullptr,
-                                        const RichOut *rich = nullptr) {
+/*
+ * The two branches of an adaptive attempt on a t-spectral field (the
+ * unnormalised t-FFT of A, the state the fixed-step schedule keeps between
+ * steps), in and out: the dispersion half steps ride on the y sweeps, so each
+ * t-line runs only its own transforms (no forward transform of its input, no
+ * inverse one of its output).  The half step before each t-line takes a 't'
+ * table (the 1/nt of the pair); the half step that ends a branch takes an 'n'
+ * table, so the branch leaves the unnormalised scaling it started from.
+ * C1 (step_once_f) is one step of 2dz; C2 (two_steps_f) two steps of dz with
+ * the middle half steps merged (their linear factors commute): x y | t | x y
+ * | t | x y, 8 sweeps instead of 10.  Signs and signmem as step_once gives
+ * them, once per step.
+ */
+template <class T> static int step_once_f(mpc_ctx *c, double dzz, double z, int *signmem, const void *src) {
+    int s;
+    if (c->p.mirror_R > 0.0) {
+        s = 1;
+        if (signmem) *signmem = 1;
+    } else {
+        s = pass_sign(c, z);
+        if (signmem && -*signmem == s) *signmem = -*signmem;
+    }
+    const cx<T> *P = get_table<T>(c, 't', dzz / 2), *Pn = get_table<T>(c, 'n', dzz / 2);
+    if (!P || !Pn) return MPC_ENOMEM;
+    RCHK(run_xy<T>(c, s * dzz / 2, false, P, src));
+    RCHK(sweep_t<T>(c, nullptr, nullptr, true, true, false, dzz));
+    return run_xy<T>(c, s * dzz / 2, false, Pn);
+}
+template <class T> static int two_steps_f(mpc_ctx *c, double dzz, double z, int *signmem, const void *src,
+                                          const RichOut *rich) {
     int s[2];
     for (int i = 0; i < 2; i++) {
         const double zi = z + i * dzz;
@@ -1060,22 +1086,18 @@ template <class T> static int two_steps(mpc_ctx *c, double dzz, double z, int *s
             if (signmem && -*signmem == s[i]) *signmem = -*signmem;
         }
     }
-    /* every table carries one 1/nt, one per forward/inverse t-FFT pair: the
-     * pair that spans the two steps gets the merged P(dz/2) Q(dz/2) = P(dz) */
-    const cx<T> *P = get_table<T>(c, 't', dzz / 2);
-    const cx<T> *P2 = get_table<T>(c, 't', dzz);
-    if (!P || !P2) return MPC_ENOMEM;
-    RCHK(run_xy<T>(c, s[0] * dzz / 2, false, nullptr, src));
-    RCHK(sweep_t<T>(c, P, nullptr, false, true, false, dzz));           /* leaves the field t-spectral */
-    RCHK(run_xy<T>(c, (s[0] + s[1]) * dzz / 2, false, P2));            /* x P(dz) on the y sweep */
-    RCHK(sweep_t<T>(c, nullptr, P, true, true, true, dzz));
-    if (!rich) return run_xy<T>(c, s[1] * dzz / 2, false);
-    /* last x sweep as usual, then the y sweep fused with the error norms,
-     * the per-line powers and the Richardson combination */
-    const cx<T> *D = get_table<T>(c, 'x', s[1] * dzz / 2);
-    if (!D) return MPC_ENOMEM;
+    const cx<T> *P = get_table<T>(c, 't', dzz / 2), *P2 = get_table<T>(c, 't', dzz);
+    const cx<T> *Pn = get_table<T>(c, 'n', dzz / 2), *D = get_table<T>(c, 'x', s[1] * dzz / 2);
+    if (!P || !P2 || !Pn || !D) return MPC_ENOMEM;
+    RCHK(run_xy<T>(c, s[0] * dzz / 2, false, P, src));
+    RCHK(sweep_t<T>(c, nullptr, nullptr, true, true, false, dzz));
+    RCHK(run_xy<T>(c, (s[0] + s[1]) * dzz / 2, false, P2));
+    RCHK(sweep_t<T>(c, nullptr, nullptr, true, true, false, dzz));
+    /* last x sweep, then the y sweep (with the closing half step's t factor)
+     * fused with the error norms, the per-line powers and the Richardson
+     * combination */
     RCHK(sweep_axis<T>(c, 0, D, false));
-    AArgs<T> a = axis_args<T>(c, 1, D, false, nullptr);
+    AArgs<T> a = axis_args<T>(c, 1, D, false, Pn);
     tic(c, 2);
     const int rc = launch_axis_richardson<T>(a, c->ny, (const cx<T> *)rich->c1, rich->err_part, rich->lp_part,
                                              c->stream);
@@ -1326,7 +1348,11 @@ void mpc::setstep_decide(double *dzz, double err, double def_err, double dzmin, 
  * with the error norms, the per-line powers of C2 (BP1/BP2 -> calc1oes, the
  * step-size cap) and R = 4/3 C2 - 1/3 C1, which it leaves in C2's buffer;
  * an accepted step adopts that buffer as the field.  One host round trip per
- * attempt brings back the two norms and the line powers.
+ * attempt brings back the two norms and the line powers.  The attempts run
+ * on the t-spectral field: the norms are sums of |.|^2 over whole lines, and
+ * the line powers feed only calc1oes's normalised profile, so Parseval makes
+ * both the same in either domain (the spectral line powers are nt times the
+ * temporal ones).
  */
 template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem, int max_steps,
                                        int *accepted, double *errs, double *dzs) {
@@ -1345,18 +1371,27 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
     RichOut ro;
     ro.err_part = c->rich_err;
     ro.lp_part = c->rich_lp;
+    bool spectral = false;
     while (*z < c->p.dist && acc < max_steps && guard < 64 * max_steps + 64) {
         guard++;
         const double dz = *dzz;
         void *A = c->field;
+        /* the attempts run on the t-spectral field: one forward t-transform
+         * before the first, one inverse after the last */
+        if (!spectral) {
+            RCHK(sweep_t<T>(c, nullptr, nullptr, false, false, false, 0.0));
+            c->domain_freq = 1;
+            spectral = true;
+            A = c->field;
+        }
         /* C1 and C2 start from A without copies: each one's first sweep reads
          * A and writes its own buffer */
         int sm1 = *signmem;
         c->field = c->aux1;
-        int rc = step_once<T>(c, 2 * dz, *z, &sm1, A);
+        int rc = step_once_f<T>(c, 2 * dz, *z, &sm1, A);
         c->field = c->aux2;
         ro.c1 = c->aux1;
-        if (!rc) rc = two_steps<T>(c, dz, *z, signmem, A, &ro);   /* sub-steps at z and z + dz */
+        if (!rc) rc = two_steps_f<T>(c, dz, *z, signmem, A, &ro);   /* sub-steps at z and z + dz */
         c->field = A;
         RCHK(rc);
         RCHK(launch_sum_pairs(c->rich_err, ntiles, c->rich_err + 2 * ntiles, c->stream));
@@ -1386,6 +1421,13 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
         if (errs) errs[acc] = err;
         if (dzs) dzs[acc] = *dzz;
         acc++;
+    }
+    if (spectral) {
+        /* back to the time domain: the 1/nt of the pair, then the inverse */
+        const cx<T> *Q0 = get_table<T>(c, 't', 0.0);
+        if (!Q0) return MPC_ENOMEM;
+        RCHK(sweep_t<T>(c, nullptr, Q0, true, false, true, 0.0));
+        c->domain_freq = 0;
     }
     if (accepted) *accepted = acc;
     return finish(c, 0);
