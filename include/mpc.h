@@ -82,7 +82,8 @@ int  mpc_download(mpc_ctx *ctx, cufftDoubleComplex *A);
 /*
  * Asynchronous checkpoint snapshot (the reference downloads and writes its
  * checkpoint synchronously, MPC.c:521-544).  mpc_snapshot_begin queues the
- * resident (time-domain) field's conversion to the reference layout and its
+ * resident field's conversion to the reference layout (a t-spectral field is
+ * first brought back to the time domain in place) and its
  * copy into a pinned host buffer the context owns, then returns at once;
  * runs started afterwards overlap the copy and do not change the snapshot.
  * mpc_snapshot_poll returns 1 once the copy is complete (or none is
@@ -117,6 +118,8 @@ int  mpc_mirror(mpc_ctx *ctx);
  * merge).  Time-domain field in and out.  No reference counterpart: the pass
  * surrogate takes it as the physics prior that its network corrects. */
 int  mpc_linear(mpc_ctx *ctx, double length, int sign, int mirror);
+/* temp, spec, BP1/BP2, energy, rms spectral width and peak rho of the resident
+ * field, in the time or the t-spectral domain as it is */
 int  mpc_diagnostics(mpc_ctx *ctx, mpc_diag *d);
 /* the resident time-domain field as two float planes (re then im, N points
  * each, point order [y][x][t], t fastest) in a device buffer, times scale;
@@ -125,7 +128,12 @@ int  mpc_export_planar(mpc_ctx *ctx, float *planes, double scale);
 int  mpc_import_planar(mpc_ctx *ctx, const float *planes, double scale);
 /* adaptive step doubling + Richardson (reference MPC.c:343-383,
  * propagation.c:616-677) for at most max_steps accepted steps.  z, dzz and
- * signmem are updated; errs/dzs (may be NULL) receive per accepted step. */
+ * signmem are updated; errs/dzs (may be NULL) receive per accepted step.
+ * The attempts run on the t-spectral field and the call leaves it so, so
+ * consecutive calls transform nothing between them; mpc_diagnostics reads
+ * either domain, and mpc_download, mpc_snapshot_begin, mpc_export_planar,
+ * mpc_step and mpc_linear first bring the field back to the time domain in
+ * place (one t-transform sweep). */
 int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int max_steps,
                       int *accepted, double *errs, double *dzs);
 

@@ -1105,9 +1105,21 @@ template <class T> static int two_steps_f(mpc_ctx *c, double dzz, double z, int 
     return rc;
 }
 
+/* a t-spectral field (left so by mpc_run_adaptive) back to the time domain,
+ * in place: the 1/nt of the transform pair, then the inverse transform */
+template <class T> static int to_time(mpc_ctx *c) {
+    if (!c->domain_freq) return 0;
+    const cx<T> *Q0 = get_table<T>(c, 't', 0.0);
+    if (!Q0) return MPC_ENOMEM;
+    RCHK(sweep_t<T>(c, nullptr, Q0, true, false, true, 0.0));
+    c->domain_freq = 0;
+    return 0;
+}
+static int to_time_any(mpc_ctx *c) { return c->prec == MPC_F32 ? to_time<float>(c) : to_time<double>(c); }
+
 extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
     if (!c) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    RCHK(to_time_any(c));
     const int rc = c->prec == MPC_F32 ? step_once<float>(c, dzz, z, signmem) : step_once<double>(c, dzz, z, signmem);
     return finish(c, rc);
 }
@@ -1126,7 +1138,7 @@ template <class T> static int linear_once(mpc_ctx *c, double length, int sign, b
 
 extern "C" int mpc_linear(mpc_ctx *c, double length, int sign, int mirror) {
     if (!c || !(length >= 0.0) || (sign != 1 && sign != -1)) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    RCHK(to_time_any(c));
     const int rc = c->prec == MPC_F32 ? linear_once<float>(c, length, sign, mirror != 0)
                                       : linear_once<double>(c, length, sign, mirror != 0);
     return finish(c, rc);
@@ -1181,7 +1193,7 @@ extern "C" int mpc_upload(mpc_ctx *c, const cufftDoubleComplex *A) {
 
 extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
     if (!c || !A) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
     RCHK(staging_guard(c));
     const int rc = c->prec == MPC_F32
         ? launch_to_reference<float>((const cx<float> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream)
@@ -1197,7 +1209,7 @@ extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
  * stream, while the caller goes on stepping. */
 extern "C" int mpc_snapshot_begin(mpc_ctx *c) {
     if (!c) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
     if (!c->copy_stream) {
         HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
@@ -1245,7 +1257,7 @@ extern "C" int mpc_snapshot_wait(mpc_ctx *c, const cufftDoubleComplex **A) {
  * internal point order [y][x][t], times scale; and back */
 extern "C" int mpc_export_planar(mpc_ctx *c, float *planes, double scale) {
     if (!c || !planes) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
     const double s = scale * c->ascale;
     const int rc = c->prec == MPC_F32 ? launch_planar<float>((cx<float> *)c->field, planes, c->N, s, 1, c->stream)
                                       : launch_planar<double>((cx<double> *)c->field, planes, c->N, s, 1, c->stream);
@@ -1279,12 +1291,14 @@ extern "C" int mpc_calibrate(mpc_ctx *c) {
 /* ------------------------------------------------------------------------ */
 extern "C" int mpc_diagnostics(mpc_ctx *c, mpc_diag *d) {
     if (!c || !d) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
+    /* either domain: a t-spectral field gives spec directly and temp and the
+     * line powers through the inverse transform */
     const int nb = c->diag_nblocks;
     double *tp = c->diag_part, *sp = c->diag_part + (size_t)nb * c->nt;
+    const bool spec = c->domain_freq != 0;
     int rc = c->prec == MPC_F32
-        ? launch_diag_lines<float>((const cx<float> *)c->field, (const cx<float> *)c->tw_t, c->nt, c->nxy, c->diag_line, tp, sp, nb, c->stream)
-        : launch_diag_lines<double>((const cx<double> *)c->field, (const cx<double> *)c->tw_t, c->nt, c->nxy, c->diag_line, tp, sp, nb, c->stream);
+        ? launch_diag_lines<float>((const cx<float> *)c->field, (const cx<float> *)c->tw_t, c->nt, c->nxy, c->diag_line, tp, sp, nb, spec, c->stream)
+        : launch_diag_lines<double>((const cx<double> *)c->field, (const cx<double> *)c->tw_t, c->nt, c->nxy, c->diag_line, tp, sp, nb, spec, c->stream);
     RCHK(rc);
     RCHK(launch_reduce_cols(tp, nb, c->nt, c->diag_out, c->stream));
     RCHK(launch_reduce_cols(sp, nb, c->nt, c->diag_out + c->nt, c->stream));
@@ -1352,7 +1366,9 @@ void mpc::setstep_decide(double *dzz, double err, double def_err, double dzmin, 
  * on the t-spectral field: the norms are sums of |.|^2 over whole lines, and
  * the line powers feed only calc1oes's normalised profile, so Parseval makes
  * both the same in either domain (the spectral line powers are nt times the
- * temporal ones).
+ * temporal ones).  The field enters in either domain and leaves t-spectral,
+ * so consecutive calls (one accepted step each, as the driver makes them)
+ * transform nothing between them.
  */
 template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem, int max_steps,
                                        int *accepted, double *errs, double *dzs) {
@@ -1371,7 +1387,7 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
     RichOut ro;
     ro.err_part = c->rich_err;
     ro.lp_part = c->rich_lp;
-    bool spectral = false;
+    bool spectral = c->domain_freq != 0;
     while (*z < c->p.dist && acc < max_steps && guard < 64 * max_steps + 64) {
         guard++;
         const double dz = *dzz;
@@ -1422,13 +1438,8 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
         if (dzs) dzs[acc] = *dzz;
         acc++;
     }
-    if (spectral) {
-        /* back to the time domain: the 1/nt of the pair, then the inverse */
-        const cx<T> *Q0 = get_table<T>(c, 't', 0.0);
-        if (!Q0) return MPC_ENOMEM;
-        RCHK(sweep_t<T>(c, nullptr, Q0, true, false, true, 0.0));
-        c->domain_freq = 0;
-    }
+    /* the field stays t-spectral for the next call; the calls that need it
+     * in time bring it back themselves (to_time) */
     if (accepted) *accepted = acc;
     return finish(c, 0);
 }
@@ -1436,7 +1447,6 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
 extern "C" int mpc_run_adaptive(mpc_ctx *c, double *z, double *dzz, int *signmem, int max_steps,
                                 int *accepted, double *errs, double *dzs) {
     if (!c || !z || !dzz || !signmem) return MPC_EINVAL;
-    if (c->domain_freq) return MPC_EINVAL;
     return c->prec == MPC_F32 ? adaptive<float>(c, z, dzz, signmem, max_steps, accepted, errs, dzs)
                               : adaptive<double>(c, z, dzz, signmem, max_steps, accepted, errs, dzs);
 }

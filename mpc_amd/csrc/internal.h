@@ -89,7 +89,7 @@ template <class T> int launch_scale(cx<T> *f, long n, T s, hipStream_t st);
 template <class T> int launch_planar(cx<T> *f, float *planes, long n, double s, int to_planes, hipStream_t st);
 template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines,
                                          double *line_power, double *temp_part, double *spec_part,
-                                         int nblocks, hipStream_t s);
+                                         int nblocks, bool spectral, hipStream_t s);
 int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s);
 template <class T> int launch_err_partials(const cx<T> *c1, const cx<T> *c2, long n, double *part,
                                            int nblocks, hipStream_t s);

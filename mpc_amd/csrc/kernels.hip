@@ -121,7 +121,10 @@ template int launch_scale<float>(cx<float> *, long, float, hipStream_t);
 /* ---------------------------------------------------------------------- */
 /* diagnostics: per-line power, and per-block partial temp[t] / spec[t]    */
 /* ---------------------------------------------------------------------- */
-template <class T, int NT>
+/* SPEC: the field holds the unnormalised t-FFT of each line (the t-spectral
+ * state); spec is then read off directly and temp and the line power come
+ * from the inverse transform, scaled by 1/nt^2 */
+template <class T, int NT, bool SPEC>
 __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> *tw, long nlines,
                                                     double *line_power, double *temp_part, double *spec_part) {
     using G = TGeo<NT>;
@@ -145,19 +148,33 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
         cx<T> v[E];
         double pw = 0.0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            v[m] = g[tid + TPL * m];
-            const double p = (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
-            at[m] += p;
-            pw += p;
+        for (int m = 0; m < E; ++m) v[m] = g[tid + TPL * m];
+        if constexpr (SPEC) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
+            line_fft<T, NT, TPlan<NT>, +1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
+            constexpr double inv2 = 1.0 / ((double)NT * (double)NT);
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const double p = ((double)v[m].re * v[m].re + (double)v[m].im * v[m].im) * inv2;
+                at[m] += p;
+                pw += p;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const double p = (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
+                at[m] += p;
+                pw += p;
+            }
+            line_fft<T, NT, TPlan<NT>, -1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
+#pragma unroll
+            for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
         }
         /* line power: reduce over the TPL threads of the line */
         constexpr int W = TPL < 64 ? TPL : 64;
         for (int o = W / 2; o > 0; o >>= 1) pw += __shfl_down(pw, o, W);
         if (tid % 64 == 0) lp[lslot][tid / 64] = pw;
-        line_fft<T, NT, TPlan<NT>, -1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
-#pragma unroll
-        for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
         __syncthreads();
         if (tid == 0) {
             double sum = 0.0;
@@ -180,25 +197,31 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
     }
 }
 template <class T, int NT>
-static int diag_go(const cx<T> *f, const cx<T> *tw, long nlines, double *lp, double *tp, double *sp, int nb, hipStream_t s) {
+static int diag_go(const cx<T> *f, const cx<T> *tw, long nlines, double *lp, double *tp, double *sp, int nb,
+                   bool spectral, hipStream_t s) {
     using G = TGeo<NT>;
     if (nlines % G::LPB) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_diag_lines<T, NT>), dim3(nb), dim3(G::THREADS), 0, s, f, tw, nlines, lp, tp, sp);
+    if (spectral)
+        hipLaunchKernelGGL((k_diag_lines<T, NT, true>), dim3(nb), dim3(G::THREADS), 0, s, f, tw, nlines, lp, tp, sp);
+    else
+        hipLaunchKernelGGL((k_diag_lines<T, NT, false>), dim3(nb), dim3(G::THREADS), 0, s, f, tw, nlines, lp, tp, sp);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T> int launch_diag_lines(const cx<T> *f, const cx<T> *tw, int nt, long nlines, double *lp,
-                                         double *tp, double *sp, int nb, hipStream_t s) {
+                                         double *tp, double *sp, int nb, bool spectral, hipStream_t s) {
     switch (nt) {
-    case 256: return diag_go<T, 256>(f, tw, nlines, lp, tp, sp, nb, s);
-    case 512: return diag_go<T, 512>(f, tw, nlines, lp, tp, sp, nb, s);
-    case 1024: return diag_go<T, 1024>(f, tw, nlines, lp, tp, sp, nb, s);
-    case 2048: return diag_go<T, 2048>(f, tw, nlines, lp, tp, sp, nb, s);
-    case 4096: return diag_go<T, 4096>(f, tw, nlines, lp, tp, sp, nb, s);
+    case 256: return diag_go<T, 256>(f, tw, nlines, lp, tp, sp, nb, spectral, s);
+    case 512: return diag_go<T, 512>(f, tw, nlines, lp, tp, sp, nb, spectral, s);
+    case 1024: return diag_go<T, 1024>(f, tw, nlines, lp, tp, sp, nb, spectral, s);
+    case 2048: return diag_go<T, 2048>(f, tw, nlines, lp, tp, sp, nb, spectral, s);
+    case 4096: return diag_go<T, 4096>(f, tw, nlines, lp, tp, sp, nb, spectral, s);
     default: return MPC_EUNSUPP;
     }
 }
-template int launch_diag_lines<double>(const cx<double> *, const cx<double> *, int, long, double *, double *, double *, int, hipStream_t);
-template int launch_diag_lines<float>(const cx<float> *, const cx<float> *, int, long, double *, double *, double *, int, hipStream_t);
+template int launch_diag_lines<double>(const cx<double> *, const cx<double> *, int, long, double *, double *, double *,
+                                       int, bool, hipStream_t);
+template int launch_diag_lines<float>(const cx<float> *, const cx<float> *, int, long, double *, double *, double *,
+                                      int, bool, hipStream_t);
 
 /* out[c] = sum_r part[r][c], in row order (deterministic) */
 __global__ void k_reduce_cols(const double *part, int nrows, int ncols, double *out) {

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from mpc_amd import Propagator, configs, make_inputs
+from mpc_amd import Propagator, configs, lib, make_inputs
+from mpc_amd._lib import check
 
 pytestmark = pytest.mark.gpu
 
@@ -186,6 +187,53 @@ def test_adaptive_across_pass_ends_matches_oracle(gpu_lib, prec, tol):
     np.testing.assert_allclose(r["errs"], errs_o[:n_o], rtol=1e-6 if prec == "f64" else 2e-2)
     assert abs(r["z"] - z_o) < 1e-15
     assert err <= tol
+
+
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_adaptive_calls_chain_and_read_either_domain(gpu_lib, prec, rtol):
+    """mpc_run_adaptive leaves the field t-spectral: two calls of 6 accepted
+    steps give the 12-step call's field bit for bit; diagnostics read in the
+    t-spectral state match those read after download has brought the field
+    back to the time domain; mpc_step continues from the t-spectral state as
+    from the downloaded and re-uploaded field"""
+    print(f"case adaptive chained calls 64x64x512 prec={prec}", flush=True)
+    wl = configs.small(64, 512, steps_per_pass=4, passes=12, mirror_R=0.3, energy=3e-4)
+    wl.def_err, wl.dzmin, wl.cell_L = 1e-3, 1e-7, 0.004
+    inp = make_inputs(wl)
+    A0 = inp.calibrated()
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        r = p.run_adaptive(0.0, inp.params.dzz, 1, 12)
+        one = p.download()
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        r1 = p.run_adaptive(0.0, inp.params.dzz, 1, 6)
+        r2 = p.run_adaptive(r1["z"], r1["dzz"], r1["signmem"], 6)
+        d_spec = p.diagnostics()
+        two = p.download()
+        d_time = p.diagnostics()
+    assert r1["accepted"] == 6 and r2["accepted"] == 6
+    assert np.array_equal(np.concatenate([r1["dzs"], r2["dzs"]]), r["dzs"])
+    assert np.array_equal(one, two)
+    for k in ("temp", "spec", "BP1", "BP2"):
+        a, b = d_spec[k], d_time[k]
+        assert np.abs(a - b).max() <= rtol * np.abs(b).max(), k
+    for k in ("energy_sum", "spec_rms_width"):
+        assert abs(d_spec[k] - d_time[k]) <= rtol * abs(d_time[k]), k
+    # mpc_step straight from the t-spectral state, and from the same field
+    # uploaded in the time domain
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run_adaptive(0.0, inp.params.dzz, 1, 12)
+        sm = C.c_int(r["signmem"])
+        check(lib().mpc_step(p.ctx, r["dzz"], r["z"], C.byref(sm)), "mpc_step")
+        stepped = p.download()
+    with Propagator(inp, prec) as q:
+        q.upload(one)
+        sm = C.c_int(r["signmem"])
+        check(lib().mpc_step(q.ctx, r["dzz"], r["z"], C.byref(sm)), "mpc_step")
+        ref = q.download()
+    assert O.rel_l2(stepped, ref) <= (1e-12 if prec == "f64" else 1e-6)
 
 
 def test_cfg1_full_pass_fp64(gpu_lib):
