@@ -270,8 +270,8 @@ k_axis_batch_p(const AArgs<T> *__restrict__ d, const long *__restrict__ tstart, 
  *   err_part[2 tile + 0/1] = sum |C2 - C1|^2, sum |C2|^2      (FP64)
  *   lp_part[chunk][y][x]   = sum over the chunk's t of |C2|^2 (FP64, t order)
  * and stores R = 4/3 C2 - 1/3 C1 (formed in FP64) in place of C2, so an
- * accepted step only has to adopt that buffer.  One workgroup per tile,
- * chunks of each x consecutive; the plan is Plan<N> (radix 16 where it
+ * accepted step only has to adopt that buffer.  Tiles with the chunks of
+ * each x consecutive; the plan is Plan<N> (radix 16 where it
  * applies) where its workgroup fits in 1024 threads, else the 8-per-thread
  * one.
  */
@@ -280,74 +280,102 @@ template <class T, int N> struct RichCfg {
     using PL = std::conditional_t<(G * (N / Plan<N>::E) <= 1024), Plan<N>, APlan8<N>>;
     using C = ACfg<T, N, PL, G>;
 };
-template <class T, int N>
+/* PERSIST: the grid is what fits on the device at once and each workgroup
+ * walks tiles blockIdx.x, + gridDim.x, ..., with its twiddle powers and D(k)
+ * entries in registers (as k_axis's persistent form); the per-tile work and
+ * outputs are the same either way */
+template <class T, int N, bool PERSIST>
 __global__ void __launch_bounds__((RichCfg<T, N>::C::THREADS))
 k_axis_richardson(AArgs<T> a, const cx<T> *__restrict__ c1, double *__restrict__ err_part,
                   double *__restrict__ lp_part, long nxy, int nx) {
     using RC = RichCfg<T, N>;
     using PL = typename RC::PL;
     using C = typename RC::C;
+    using CO = ChainOf<T, N, C::E, typename PL::R>;
     constexpr int E = C::E, TPL = C::TPL, G = C::G, THREADS = C::THREADS;
     __shared__ __align__(16) cx<T> lds[N * G + G];   /* G more: the padded line-power rows below */
     constexpr int W = THREADS < 64 ? THREADS : 64, NWV = THREADS / W;   /* lanes per wave, waves */
     __shared__ double wred[2 * NWV];
-    const int gcol = threadIdx.x % G, tid = threadIdx.x / G;
-    const long lin = blockIdx.x;
-    long outer, t0;
-    axis_tile_of(a, lin, outer, t0, G);
-    const long off0 = outer * a.s_outer + t0 + gcol;
-    const cx<T> *in = a.src ? a.src : a.field;
-    cx<T> v[E], Dc[E];
+    const int gcol0 = threadIdx.x % G, tid0 = threadIdx.x / G;
+    const long ntiles = (long)(a.nt / G) * a.n_outer;
+    std::conditional_t<PERSIST, typename CO::type, NoChain> twc;
+    cx<T> Dc[E];
+    if constexpr (PERSIST) {
+        twc.load(a.tw, tid0);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = gload(in + off0, (long)(tid + TPL * m) * a.s_line);
-    const cx<T> p = a.Pt ? gload(a.Pt, t0 + gcol) : mk((T)1, (T)0);
+        for (int m = 0; m < E; ++m) Dc[m] = gload(a.D, tid0 + TPL * m);
+    }
+    int tid = tid0, gcol = gcol0;
     auto fft = [&](cx<T>(&x)[E], auto dir) {
         constexpr int DIR = decltype(dir)::value;
-        line_fft<T, N, PL, DIR>(x, tid, lds + gcol, G, a.tw);
+        if constexpr (PERSIST) CO::template fft<DIR, 0>(x, tid, lds + gcol, G, twc);
+        else line_fft<T, N, PL, DIR>(x, tid, lds + gcol, G, a.tw);
     };
-    axis_apply<T, N, PL, G, false>(a, v, p, outer, tid, Dc, fft);
-    __syncthreads();                                  /* the transform's last LDS reads are done */
-    /* |C2|^2 per (t in chunk, line along y), rows padded to N + 1 doubles so
-     * that both the writes (consecutive lanes: consecutive t) and the reads
-     * (consecutive lanes: consecutive lines) are free of bank conflicts */
-    constexpr int PW = N + 1;
-    static_assert(G * PW * (int)sizeof(double) <= (N * G + G) * (int)sizeof(cx<T>),
-                  "line-power scratch must fit in the transform's LDS");
-    double *pw_t = reinterpret_cast<double *>(lds);
-    double e1 = 0.0, e2 = 0.0;
+    const cx<T> *in = a.src ? a.src : a.field;
+    for (long lin = blockIdx.x; lin < ntiles; lin += PERSIST ? (long)gridDim.x : ntiles) {
+        tid = tid0;
+        gcol = gcol0;
+        if constexpr (PERSIST) {
+            /* index arithmetic is recomputed where it is used instead of
+             * being held across the tile loop */
+            asm volatile("" : "+v"(tid), "+v"(gcol));
+            opaque_chain(twc);
+        }
+        long outer, t0;
+        axis_tile_of(a, lin, outer, t0, G);
+        const long off0 = outer * a.s_outer + t0 + gcol;
+        cx<T> v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const long off = (long)(tid + TPL * m) * a.s_line;
-        const cx<T> u = gload(c1 + off0, off);
-        const double cr = (double)v[m].re, ci = (double)v[m].im, ur = (double)u.re, ui = (double)u.im;
-        const double dr = cr - ur, di = ci - ui, pw = cr * cr + ci * ci;
-        e1 += dr * dr + di * di;
-        e2 += pw;
-        pw_t[gcol * PW + tid + TPL * m] = pw;
-        gstore(a.field + off0, off, mk((T)((4.0 / 3.0) * cr - (1.0 / 3.0) * ur), (T)((4.0 / 3.0) * ci - (1.0 / 3.0) * ui)));
-    }
-    /* block sums of the two error norms: waves, then the wave totals in order */
-    for (int o = W / 2; o > 0; o >>= 1) {
-        e1 += __shfl_down(e1, o, W);
-        e2 += __shfl_down(e2, o, W);
-    }
-    if (threadIdx.x % W == 0) {
-        wred[2 * (threadIdx.x / W)] = e1;
-        wred[2 * (threadIdx.x / W) + 1] = e2;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s1 = 0.0, s2 = 0.0;
-        for (int w = 0; w < NWV; ++w) { s1 += wred[2 * w]; s2 += wred[2 * w + 1]; }
-        err_part[2 * lin] = s1;
-        err_part[2 * lin + 1] = s2;
-    }
-    /* per-line power over the chunk's t values, in t order: line y at x = outer */
-    const long chunk = t0 / G;
-    for (int j = threadIdx.x; j < N; j += THREADS) {
-        double sum = 0.0;
-        for (int g = 0; g < G; ++g) sum += pw_t[g * PW + j];
-        lp_part[chunk * nxy + (long)j * nx + outer] = sum;
+        for (int m = 0; m < E; ++m) v[m] = gload(in + off0, (long)(tid + TPL * m) * a.s_line);
+        const cx<T> p = a.Pt ? gload(a.Pt, t0 + gcol) : mk((T)1, (T)0);
+        axis_apply<T, N, PL, G, PERSIST>(a, v, p, outer, tid, Dc, fft);
+        __syncthreads();                              /* the transform's last LDS reads are done */
+        /* |C2|^2 per (t in chunk, line along y), rows padded to N + 1 doubles
+         * so that both the writes (consecutive lanes: consecutive t) and the
+         * reads (consecutive lanes: consecutive lines) are free of bank
+         * conflicts */
+        constexpr int PW = N + 1;
+        static_assert(G * PW * (int)sizeof(double) <= (N * G + G) * (int)sizeof(cx<T>),
+                      "line-power scratch must fit in the transform's LDS");
+        double *pw_t = reinterpret_cast<double *>(lds);
+        double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const long off = (long)(tid + TPL * m) * a.s_line;
+            const cx<T> u = gload(c1 + off0, off);
+            const double cr = (double)v[m].re, ci = (double)v[m].im, ur = (double)u.re, ui = (double)u.im;
+            const double dr = cr - ur, di = ci - ui, pw = cr * cr + ci * ci;
+            e1 += dr * dr + di * di;
+            e2 += pw;
+            pw_t[gcol * PW + tid + TPL * m] = pw;
+            gstore(a.field + off0, off,
+                   mk((T)((4.0 / 3.0) * cr - (1.0 / 3.0) * ur), (T)((4.0 / 3.0) * ci - (1.0 / 3.0) * ui)));
+        }
+        /* block sums of the two error norms: waves, then the wave totals in order */
+        for (int o = W / 2; o > 0; o >>= 1) {
+            e1 += __shfl_down(e1, o, W);
+            e2 += __shfl_down(e2, o, W);
+        }
+        if (threadIdx.x % W == 0) {
+            wred[2 * (threadIdx.x / W)] = e1;
+            wred[2 * (threadIdx.x / W) + 1] = e2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int w = 0; w < NWV; ++w) { s1 += wred[2 * w]; s2 += wred[2 * w + 1]; }
+            err_part[2 * lin] = s1;
+            err_part[2 * lin + 1] = s2;
+        }
+        /* per-line power over the chunk's t values, in t order: line y at x = outer */
+        const long chunk = t0 / G;
+        for (int j = threadIdx.x; j < N; j += THREADS) {
+            double sum = 0.0;
+            for (int g = 0; g < G; ++g) sum += pw_t[g * PW + j];
+            lp_part[chunk * nxy + (long)j * nx + outer] = sum;
+        }
+        /* wred and pw_t are read above; the next tile's transform writes lds */
+        if constexpr (PERSIST) __syncthreads();
     }
 }
 template <class T, int N>
@@ -359,8 +387,20 @@ static int rich_go(AArgs<T> a, int nx, const cx<T> *c1, double *err_part, double
     a.tg = nch;                                        /* the chunks of one x consecutive */
     const long nb = (long)nch * nx;
     if (nb <= 0 || nb > 0x7fffffffL) return MPC_EINVAL;
-    hipLaunchKernelGGL((k_axis_richardson<T, N>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a, c1, err_part, lp_part,
-                       (long)nx * N, nx);
+    /* FP32 walks the tiles with resident workgroups (0.87 against 0.97 ms at
+     * 256^2 x 2048); in FP64 the cached tables cost the form its second wave
+     * per SIMD (1.71 against 1.52 ms), so it keeps one workgroup per tile */
+    if constexpr (sizeof(T) == 4) {
+        static long resident = -1;                    /* per instantiation: the same on every device of one model */
+        if (resident < 0) resident = resident_blocks((const void *)k_axis_richardson<T, N, true>, C::THREADS, 0);
+        if (resident > 0) {
+            hipLaunchKernelGGL((k_axis_richardson<T, N, true>), dim3((unsigned)std::min(nb, resident)),
+                               dim3(C::THREADS), 0, s, a, c1, err_part, lp_part, (long)nx * N, nx);
+            return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
+        }
+    }
+    hipLaunchKernelGGL((k_axis_richardson<T, N, false>), dim3((unsigned)nb), dim3(C::THREADS), 0, s, a, c1, err_part,
+                       lp_part, (long)nx * N, nx);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 template <class T>
