@@ -83,7 +83,8 @@ int  mpc_download(mpc_ctx *ctx, cufftDoubleComplex *A);
  * Asynchronous checkpoint snapshot (the reference downloads and writes its
  * checkpoint synchronously, MPC.c:521-544).  mpc_snapshot_begin queues the
  * resident field's conversion to the reference layout (a t-spectral field is
- * first brought back to the time domain in place) and its
+ * first brought back to the time domain in place, so a run resumed from the
+ * snapshot and the uninterrupted run go on from the same field) and its
  * copy into a pinned host buffer the context owns, then returns at once;
  * runs started afterwards overlap the copy and do not change the snapshot.
  * mpc_snapshot_poll returns 1 once the copy is complete (or none is
@@ -102,8 +103,11 @@ int  mpc_calibrate(mpc_ctx *ctx);
  * Fixed-step run of nsteps z-steps starting at global step index step0:
  * each step is L(dzz/2) N(dzz) L(dzz/2) evaluated at z = (n + 1/2) dzz,
  * with the mirror applied between passes.  Consecutive half steps are merged
- * (3 HBM sweeps per interior step).  On return the field is in the time
- * domain again.
+ * (3 HBM sweeps per interior step).  On return the field is t-spectral (the
+ * state it keeps between steps), so a following run or mpc_diagnostics
+ * starts without a transform.  mpc_download and mpc_export_planar read a
+ * transformed copy and leave the field as it is; mpc_snapshot_begin,
+ * mpc_step and mpc_linear first bring it back to the time domain in place.
  */
 int  mpc_run(mpc_ctx *ctx, int step0, int nsteps);
 /* One isolated split step at z with step dzz (reference ssfmprop semantics,
@@ -129,11 +133,8 @@ int  mpc_import_planar(mpc_ctx *ctx, const float *planes, double scale);
 /* adaptive step doubling + Richardson (reference MPC.c:343-383,
  * propagation.c:616-677) for at most max_steps accepted steps.  z, dzz and
  * signmem are updated; errs/dzs (may be NULL) receive per accepted step.
- * The attempts run on the t-spectral field and the call leaves it so, so
- * consecutive calls transform nothing between them; mpc_diagnostics reads
- * either domain, and mpc_download, mpc_snapshot_begin, mpc_export_planar,
- * mpc_step and mpc_linear first bring the field back to the time domain in
- * place (one t-transform sweep). */
+ * The attempts run on the t-spectral field and the call leaves it so, as
+ * mpc_run does. */
 int  mpc_run_adaptive(mpc_ctx *ctx, double *z, double *dzz, int *signmem, int max_steps,
                       int *accepted, double *errs, double *dzs);
 

@@ -422,6 +422,7 @@ struct SweepOp {
     double coef = 0.0;
     bool mirror = false;
     bool has_p = false, has_q = false;
+    bool q_unnorm = false;        /* Q without the 1/nt: the field stays t-spectral after the t-line */
     double pcoef = 0.0;
     bool in_freq = false, out_time = false;
     double dz = 0.0;
@@ -433,7 +434,11 @@ struct SweepOp {
  * L(dz/2) N(dz) L(dz/2) at z = (n + 1/2) dz; consecutive half steps merge,
  * the field stays t-spectral between steps (the dispersion factor then rides
  * on the y sweep), and the mirror is applied after the last step of each
- * pass but the last.  *freq_end receives the domain the run leaves.
+ * pass but the last.  The run also ends t-spectral (the last t-line applies
+ * its closing half step with an 'n' table and no inverse transform), so the
+ * next run, or the next diagnostics, starts without a transform; the calls
+ * that need the time domain convert on their own.  *freq_end receives the
+ * domain the run leaves.
  */
 static void plan_fixed(const mpc_ctx *c, int step0, int nsteps, std::vector<SweepOp> &ops, bool *freq_end) {
     const double dz = c->p.dzz;
@@ -466,13 +471,12 @@ static void plan_fixed(const mpc_ctx *c, int step0, int nsteps, std::vector<Swee
         t.kind = 't'; t.step = n; t.dz = dz; t.in_freq = freq;
         t.has_p = !p_in_y; t.pcoef = ct;
         if (last) {
-            t.has_q = true; t.out_time = true;
-            freq = false;
+            t.has_q = true; t.q_unnorm = true;
             ct = 0.0;
         } else {
-            freq = true;
             ct = dz / 2;
         }
+        freq = true;
         ops.push_back(t);
         cxy = s * dz / 2;
         if (mirror_after) {
@@ -488,7 +492,7 @@ static void plan_fixed(const mpc_ctx *c, int step0, int nsteps, std::vector<Swee
 template <class T> static int exec_op(mpc_ctx *c, const SweepOp &o) {
     if (o.kind == 't') {
         const cx<T> *P = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
-        const cx<T> *Q = o.has_q ? get_table<T>(c, 't', o.dz / 2) : nullptr;
+        const cx<T> *Q = o.has_q ? get_table<T>(c, o.q_unnorm ? 'n' : 't', o.dz / 2) : nullptr;
         if ((o.has_p && !P) || (o.has_q && !Q)) return MPC_ENOMEM;
         return sweep_t<T>(c, P, Q, o.in_freq, true, o.out_time, o.dz);
     }
@@ -603,7 +607,7 @@ static int batch_group(mpc_batch *b, const std::vector<std::vector<SweepOp>> &pl
                 mpc_ctx *c = b->ctx[p];
                 const SweepOp &o = plans[p][i];
                 const cx<T> *P = o.has_p ? get_table<T>(c, 't', o.pcoef) : nullptr;
-                const cx<T> *Q = o.has_q ? get_table<T>(c, 't', o.dz / 2) : nullptr;
+                const cx<T> *Q = o.has_q ? get_table<T>(c, o.q_unnorm ? 'n' : 't', o.dz / 2) : nullptr;
                 if ((o.has_p && !P) || (o.has_q && !Q)) return MPC_ENOMEM;
                 d.push_back(tline_args<T>(c, P, Q, o.in_freq, true, o.out_time, o.dz));
                 starts.push_back(tot);
@@ -1105,17 +1109,40 @@ template <class T> static int two_steps_f(mpc_ctx *c, double dzz, double z, int 
     return rc;
 }
 
-/* a t-spectral field (left so by mpc_run_adaptive) back to the time domain,
- * in place: the 1/nt of the transform pair, then the inverse transform */
+/* a t-spectral field (left so by mpc_run or mpc_run_adaptive) back to the
+ * time domain, in place: the 1/nt of the transform pair, then the inverse
+ * transform.  Always with the per-group t-line form, whatever variant the
+ * context is tuned to (in FP64 the forms' twiddle products round
+ * differently), so the time-domain field does not depend on the tuning. */
 template <class T> static int to_time(mpc_ctx *c) {
     if (!c->domain_freq) return 0;
     const cx<T> *Q0 = get_table<T>(c, 't', 0.0);
     if (!Q0) return MPC_ENOMEM;
-    RCHK(sweep_t<T>(c, nullptr, Q0, true, false, true, 0.0));
+    const int keep = c->tvar;
+    c->tvar = 1;
+    const int rc = sweep_t<T>(c, nullptr, Q0, true, false, true, 0.0);
+    c->tvar = keep;
+    RCHK(rc);
     c->domain_freq = 0;
     return 0;
 }
 static int to_time_any(mpc_ctx *c) { return c->prec == MPC_F32 ? to_time<float>(c) : to_time<double>(c); }
+/* fn on a time-domain view of c's field that leaves c as it was (a read: a
+ * t-spectral field is copied to aux1 and transformed there), so reading the
+ * field never changes the path a later run takes */
+template <class F> static int time_view(mpc_ctx *c, F &&fn) {
+    if (!c->domain_freq) return fn();
+    const size_t bytes = c->esz * (size_t)c->N;
+    if (!c->aux1 && hipMalloc(&c->aux1, bytes) != hipSuccess) return MPC_ENOMEM;
+    HIPCHK(hipMemcpyAsync(c->aux1, c->field, bytes, hipMemcpyDeviceToDevice, c->stream));
+    void *keep = c->field;
+    c->field = c->aux1;
+    int rc = to_time_any(c);
+    if (!rc) rc = fn();
+    c->field = keep;
+    c->domain_freq = 1;
+    return rc;
+}
 
 extern "C" int mpc_step(mpc_ctx *c, double dzz, double z, int *signmem) {
     if (!c) return MPC_EINVAL;
@@ -1193,12 +1220,12 @@ extern "C" int mpc_upload(mpc_ctx *c, const cufftDoubleComplex *A) {
 
 extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
     if (!c || !A) return MPC_EINVAL;
-    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
     RCHK(staging_guard(c));
-    const int rc = c->prec == MPC_F32
-        ? launch_to_reference<float>((const cx<float> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream)
-        : launch_to_reference<double>((const cx<double> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream);
-    RCHK(rc);
+    RCHK(time_view(c, [&] {
+        return c->prec == MPC_F32
+            ? launch_to_reference<float>((const cx<float> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream)
+            : launch_to_reference<double>((const cx<double> *)c->field, c->staging, c->nx, c->ny, c->nt, c->ascale, c->stream);
+    }));
     HIPCHK(hipMemcpyAsync(A, c->staging, 16 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
     return finish(c, 0);
 }
@@ -1209,7 +1236,9 @@ extern "C" int mpc_download(mpc_ctx *c, cufftDoubleComplex *A) {
  * stream, while the caller goes on stepping. */
 extern "C" int mpc_snapshot_begin(mpc_ctx *c) {
     if (!c) return MPC_EINVAL;
-    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
+    /* in place: an interrupted run and the run resumed from this snapshot
+     * then go on from the same time-domain field */
+    RCHK(to_time_any(c));
     if (!c->copy_stream) {
         HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
@@ -1257,10 +1286,11 @@ extern "C" int mpc_snapshot_wait(mpc_ctx *c, const cufftDoubleComplex **A) {
  * internal point order [y][x][t], times scale; and back */
 extern "C" int mpc_export_planar(mpc_ctx *c, float *planes, double scale) {
     if (!c || !planes) return MPC_EINVAL;
-    RCHK(to_time_any(c));      /* in place, if mpc_run_adaptive left it t-spectral */
     const double s = scale * c->ascale;
-    const int rc = c->prec == MPC_F32 ? launch_planar<float>((cx<float> *)c->field, planes, c->N, s, 1, c->stream)
-                                      : launch_planar<double>((cx<double> *)c->field, planes, c->N, s, 1, c->stream);
+    const int rc = time_view(c, [&] {
+        return c->prec == MPC_F32 ? launch_planar<float>((cx<float> *)c->field, planes, c->N, s, 1, c->stream)
+                                  : launch_planar<double>((cx<double> *)c->field, planes, c->N, s, 1, c->stream);
+    });
     return finish(c, rc);
 }
 extern "C" int mpc_import_planar(mpc_ctx *c, const float *planes, double scale) {

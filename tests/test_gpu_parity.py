@@ -193,9 +193,9 @@ def test_adaptive_across_pass_ends_matches_oracle(gpu_lib, prec, tol):
 def test_adaptive_calls_chain_and_read_either_domain(gpu_lib, prec, rtol):
     """mpc_run_adaptive leaves the field t-spectral: two calls of 6 accepted
     steps give the 12-step call's field bit for bit; diagnostics read in the
-    t-spectral state match those read after download has brought the field
-    back to the time domain; mpc_step continues from the t-spectral state as
-    from the downloaded and re-uploaded field"""
+    t-spectral state match those of the downloaded field uploaded into a
+    fresh context (time domain); mpc_step continues from the t-spectral state
+    as from the downloaded and re-uploaded field"""
     print(f"case adaptive chained calls 64x64x512 prec={prec}", flush=True)
     wl = configs.small(64, 512, steps_per_pass=4, passes=12, mirror_R=0.3, energy=3e-4)
     wl.def_err, wl.dzmin, wl.cell_L = 1e-3, 1e-7, 0.004
@@ -211,7 +211,10 @@ def test_adaptive_calls_chain_and_read_either_domain(gpu_lib, prec, rtol):
         r2 = p.run_adaptive(r1["z"], r1["dzz"], r1["signmem"], 6)
         d_spec = p.diagnostics()
         two = p.download()
-        d_time = p.diagnostics()
+        assert np.array_equal(p.download(), two)        # a download leaves the field as it is
+    with Propagator(inp, prec) as q:                    # the same field, uploaded in the time domain
+        q.upload(two)
+        d_time = q.diagnostics()
     assert r1["accepted"] == 6 and r2["accepted"] == 6
     assert np.array_equal(np.concatenate([r1["dzs"], r2["dzs"]]), r["dzs"])
     assert np.array_equal(one, two)
@@ -234,6 +237,40 @@ def test_adaptive_calls_chain_and_read_either_domain(gpu_lib, prec, rtol):
         check(lib().mpc_step(q.ctx, r["dzz"], r["z"], C.byref(sm)), "mpc_step")
         ref = q.download()
     assert O.rel_l2(stepped, ref) <= (1e-12 if prec == "f64" else 1e-6)
+
+
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_fixed_runs_end_t_spectral_and_chain(gpu_lib, prec, rtol):
+    """mpc_run leaves the field t-spectral: diagnostics and energy read there
+    match those of the downloaded field uploaded into a fresh context (time
+    domain); two runs of 3 steps match one run of 6 (the half steps between
+    them unmerged)"""
+    print(f"case fixed runs chained 32x32x512 prec={prec}", flush=True)
+    inp, A0 = _case(32, 512)
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, 6)
+        one = p.download()
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        p.run(0, 3)
+        p.run(3, 3)
+        d_spec = p.diagnostics()
+        e_spec = C.c_double()
+        check(lib().mpc_energy(p.ctx, C.byref(e_spec)), "mpc_energy")
+        two = p.download()
+    with Propagator(inp, prec) as q:                    # the same field, uploaded in the time domain
+        q.upload(two)
+        d_time = q.diagnostics()
+        e_time = C.c_double()
+        check(lib().mpc_energy(q.ctx, C.byref(e_time)), "mpc_energy")
+    assert O.rel_l2(two, one) <= (1e-12 if prec == "f64" else 1e-5)
+    for k in ("temp", "spec", "BP1", "BP2"):
+        a, b = d_spec[k], d_time[k]
+        assert np.abs(a - b).max() <= rtol * np.abs(b).max(), k
+    for k in ("energy_sum", "spec_rms_width"):
+        assert abs(d_spec[k] - d_time[k]) <= rtol * abs(d_time[k]), k
+    assert abs(e_spec.value - e_time.value) <= rtol * e_time.value
 
 
 def test_cfg1_full_pass_fp64(gpu_lib):
