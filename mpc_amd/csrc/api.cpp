@@ -201,7 +201,10 @@ extern "C" int mpc_create(const mpc_params *p, int precision, mpc_ctx **out) {
         if (hipMalloc((void **)&c->rho_block, sizeof(double) * (size_t)c->rho_nblocks) != hipSuccess) rc = MPC_ENOMEM;
     }
     if (!rc) {
-        c->diag_nblocks = 512;
+        /* blocks of the diagnostics pass: one per line group up to 1024 (a
+         * group is max(1, 2048 / nt) lines) */
+        const long dgroups = c->nxy / std::max(1, 2048 / c->nt);
+        c->diag_nblocks = (int)std::max(1L, std::min(1024L, dgroups));
         if (hipMalloc((void **)&c->diag_part, sizeof(double) * 2 * (size_t)c->diag_nblocks * c->nt) != hipSuccess ||
             hipMalloc((void **)&c->diag_line, sizeof(double) * (size_t)c->nxy) != hipSuccess ||
             hipMalloc((void **)&c->diag_out, sizeof(double) * 2 * (size_t)c->nt) != hipSuccess ||

@@ -129,30 +129,50 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
                                                     double *line_power, double *temp_part, double *spec_part) {
     using G = TGeo<NT>;
     constexpr int E = G::E, TPL = G::TPL, LPB = G::LPB;
+    /* with one line per block each thread owns its t values and writes its
+     * partial sums itself; with several, the slots are combined in LDS */
+    constexpr bool SLOTS = LPB > 1;
     __shared__ __align__(16) cx<T> lds[LPB * G::FFT_ELEMS];
-    __shared__ double accT[NT], accS[NT];
+    __shared__ double accT[SLOTS ? NT : 1], accS[SLOTS ? NT : 1];
     __shared__ double lp[LPB][TPL / 64 + 1];
     const int lslot = threadIdx.x / TPL, tid = threadIdx.x % TPL;
-    for (int i = threadIdx.x; i < NT; i += blockDim.x) { accT[i] = 0.0; accS[i] = 0.0; }
+    if constexpr (SLOTS)
+        for (int i = threadIdx.x; i < NT; i += blockDim.x) { accT[i] = 0.0; accS[i] = 0.0; }
     /* the zeroing threads are not the ones that add each slot below, and a
      * block with no line group reaches no barrier inside the loop */
     __syncthreads();
     double at[E], as[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) { at[m] = 0.0; as[m] = 0.0; }
-    /* a fixed set of line groups per block: deterministic order */
+    /* a fixed set of line groups per block: deterministic order; the next
+     * group's line is loaded while this one is transformed */
     const long ngroups = nlines / LPB;
+    cx<T> nxt[E];
+    if (blockIdx.x < ngroups) {
+        const cx<T> *g0 = f + ((long)blockIdx.x * LPB + lslot) * NT;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = g0[tid + TPL * m];
+    }
     for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const long line = grp * LPB + lslot;
-        const cx<T> *g = f + line * NT;
+        /* the table pointer re-enters as an opaque value each iteration, so
+         * the stage twiddles load where they are used instead of being
+         * hoisted out of the loop and held in registers throughout */
+        const cx<T> *twi = tw;
+        asm volatile("" : "+s"(twi));
         cx<T> v[E];
         double pw = 0.0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = g[tid + TPL * m];
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        if (grp + gridDim.x < ngroups) {
+            const cx<T> *gn = f + ((grp + gridDim.x) * LPB + lslot) * NT;
+#pragma unroll
+            for (int m = 0; m < E; ++m) nxt[m] = gn[tid + TPL * m];
+        }
         if constexpr (SPEC) {
 #pragma unroll
             for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
-            line_fft<T, NT, TPlan<NT>, +1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
+            line_fft<T, NT, TPlan<NT>, +1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, twi);
             constexpr double inv2 = 1.0 / ((double)NT * (double)NT);
 #pragma unroll
             for (int m = 0; m < E; ++m) {
@@ -167,7 +187,7 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
                 at[m] += p;
                 pw += p;
             }
-            line_fft<T, NT, TPlan<NT>, -1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, tw);
+            line_fft<T, NT, TPlan<NT>, -1, G::PAD>(v, tid, lds + lslot * G::FFT_ELEMS, 1, twi);
 #pragma unroll
             for (int m = 0; m < E; ++m) as[m] += (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
         }
@@ -182,6 +202,14 @@ __global__ void __launch_bounds__(256) k_diag_lines(const cx<T> *f, const cx<T> 
             line_power[line] = sum;
         }
         __syncthreads();
+    }
+    if constexpr (!SLOTS) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            temp_part[(long)blockIdx.x * NT + tid + TPL * m] = at[m];
+            spec_part[(long)blockIdx.x * NT + tid + TPL * m] = as[m];
+        }
+        return;
     }
     /* combine the LPB line slots in slot order */
     for (int sl = 0; sl < LPB; ++sl) {
@@ -223,16 +251,30 @@ template int launch_diag_lines<double>(const cx<double> *, const cx<double> *, i
 template int launch_diag_lines<float>(const cx<float> *, const cx<float> *, int, long, double *, double *, double *,
                                       int, bool, hipStream_t);
 
-/* out[c] = sum_r part[r][c], in row order (deterministic) */
-__global__ void k_reduce_cols(const double *part, int nrows, int ncols, double *out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
+/* out[c] = sum_r part[r][c], in a fixed order (deterministic): RC_PARTS
+ * threads per column sum rows q, q + RC_PARTS, ... in turn, then the parts
+ * are added in q order; consecutive threads read consecutive columns */
+constexpr int RC_COLS = 64, RC_PARTS = 16;
+__global__ void __launch_bounds__(RC_COLS * RC_PARTS) k_reduce_cols(const double *part, int nrows, int ncols,
+                                                                    double *out) {
+    __shared__ double ps[RC_PARTS][RC_COLS];
+    const int cl = threadIdx.x % RC_COLS, q = threadIdx.x / RC_COLS;
+    const long c = (long)blockIdx.x * RC_COLS + cl;
     double s = 0.0;
-    for (int r = 0; r < nrows; ++r) s += part[(long)r * ncols + c];
-    out[c] = s;
+    if (c < ncols)
+        for (int r = q; r < nrows; r += RC_PARTS) s += part[(long)r * ncols + c];
+    ps[q][cl] = s;
+    __syncthreads();
+    if (q == 0 && c < ncols) {
+        double t = 0.0;
+        for (int k = 0; k < RC_PARTS; ++k) t += ps[k][cl];
+        out[c] = t;
+    }
 }
 int launch_reduce_cols(const double *part, int nrows, int ncols, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + 255) / 256), dim3(256), 0, s, part, nrows, ncols, out);
+    if (nrows <= 0 || ncols <= 0) return MPC_EINVAL;
+    hipLaunchKernelGGL(k_reduce_cols, dim3((ncols + RC_COLS - 1) / RC_COLS), dim3(RC_COLS * RC_PARTS), 0, s, part,
+                       nrows, ncols, out);
     return hipGetLastError() == hipSuccess ? 0 : MPC_EHIP;
 }
 
