@@ -6,6 +6,8 @@
   pulse energy, rms spectral width and peak electron density.
 * The whole first pass (500 steps) in FP32 against the FP64 path: within
   1e-4 rel L2.
+* Adaptive mode at 256x256x2048: the first accepted step against the
+  oracle's (error ratio, next dz, z and the Richardson field).
 * The 48-pulse batched workload's three grid classes (128^2 x 512,
   192^2 x 1024, 256^2 x 2048): one pulse of each, steps 298-300 across the
   mirror after step 299, run as one batch; each pulse equals its own
@@ -62,6 +64,54 @@ def test_production_256_across_the_first_mirror(gpu_lib, prec):
         got = p.download()
         d = p.diagnostics()
     _check(prec, got, d, ref, rho_ref, ob, "256x256x2048")
+
+
+_adaptive = {}
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_production_256_adaptive_first_accepted_step(gpu_lib, prec):
+    """adaptive step doubling + Richardson at 256x256x2048 from the calibrated
+    input: the first accepted step (its error ratio, the next dz, z and the
+    Richardson field) against the oracle's.  The dz decision is compared where
+    the oracle's error ratio is more than 5% from every decision threshold
+    (0.5, 1 and 2 times def_err).  Here the first attempt's error is 4e-8 of
+    the field, far below def_err, which FP64 resolves and FP32 reads at its
+    rounding floor."""
+    import ctypes as C
+    wl = configs.cfg2_production_256()
+    print(f"case 256x256x2048 adaptive first accepted step prec={prec}", flush=True)
+    inp = make_inputs(wl)
+    A0 = inp.calibrated()
+    if "r" not in _adaptive:
+        po = O.params_from(inp)
+        ref = A0.copy()
+        z, dz, sm = C.c_double(0.0), C.c_double(inp.params.dzz), C.c_int(1)
+        errs, dzs = np.zeros(1), np.zeros(1)
+        n = O.oracle().oracle_run_adaptive(O.vp(ref), C.byref(po), C.byref(z), C.byref(dz), C.byref(sm), 1,
+                                           O.dp(errs), O.dp(dzs))
+        _adaptive["r"] = (ref, z.value, n, errs, dzs)
+    ref, z_o, n_o, errs_o, dzs_o = _adaptive["r"]
+    assert n_o == 1
+    with Propagator(inp, prec) as p:
+        p.upload(A0)
+        r = p.run_adaptive(0.0, inp.params.dzz, 1, 1)
+        got = p.download()
+    err = O.rel_l2(got, ref)
+    ratio = errs_o[0] / wl.def_err
+    print(f"  accepted {r['accepted']} z={r['z']:.6g}/{z_o:.6g} err ratio {r['errs'][:1] / wl.def_err}/{ratio:.4f} "
+          f"rel_l2={err:.3e}", flush=True)
+    assert r["accepted"] == 1
+    assert abs(r["z"] - z_o) < 1e-15
+    if prec == "f64":
+        np.testing.assert_allclose(r["errs"], errs_o[:1], rtol=1e-6)
+    else:
+        # the FP32 error norm resolves |C2 - C1| down to its own rounding,
+        # about 1e-6 of the field: below that it reads that floor
+        assert abs(r["errs"][0] - errs_o[0]) <= 2e-2 * errs_o[0] + 1e-6
+    if min(abs(ratio / t - 1.0) for t in (0.5, 1.0, 2.0)) > 0.05:
+        np.testing.assert_allclose(r["dzs"], dzs_o[:1], rtol=1e-12)
+    assert err <= (1e-10 if prec == "f64" else 1e-4)
 
 
 def _class_pulses():
