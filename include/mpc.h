@@ -119,13 +119,15 @@ int  mpc_mirror(mpc_ctx *ctx);
  * absorption on the t-lines, diffraction with coefficient sign * length on x
  * and y, then the mirror if `mirror` != 0.  Equals the fixed-step schedule of
  * mpc_run with the nonlinear step removed (its linear factors commute and
- * merge).  Time-domain field in and out.  No reference counterpart: the pass
- * surrogate takes it as the physics prior that its network corrects. */
+ * merge).  Time-domain field in (a t-spectral one is brought back first)
+ * and out.  No reference counterpart: the pass surrogate takes it as the
+ * physics prior that its network corrects. */
 int  mpc_linear(mpc_ctx *ctx, double length, int sign, int mirror);
 /* temp, spec, BP1/BP2, energy, rms spectral width and peak rho of the resident
  * field, in the time or the t-spectral domain as it is */
 int  mpc_diagnostics(mpc_ctx *ctx, mpc_diag *d);
-/* the resident time-domain field as two float planes (re then im, N points
+/* the resident field, in the time domain (a t-spectral one is read through a
+ * transformed copy and left as it is), as two float planes (re then im, N points
  * each, point order [y][x][t], t fastest) in a device buffer, times scale;
  * and the reverse (the pass surrogate hands its prediction back this way) */
 int  mpc_export_planar(mpc_ctx *ctx, float *planes, double scale);
