@@ -1355,7 +1355,13 @@ extern "C" int mpc_diagnostics(mpc_ctx *c, mpc_diag *d) {
     }
     d->spec_rms_width = s0 > 0 ? std::sqrt(std::max(0.0, s2 / s0 - (s1 / s0) * (s1 / s0))) : 0.0;
     /* BP1[x] = sum_{y,t}, BP2[y] = sum_{x,t} (propagation.c:343-354) */
-    if (d->BP1) for (int i = 0; i < c->nx; i++) { double s = 0; for (int j = 0; j < c->ny; j++) s += lp[(size_t)j * c->nx + i]; d->BP1[i] = s; }
+    /* BP1 row by row (each BP1[i] still adds its lines in j order, as a
+     * column walk would, with contiguous reads) */
+    if (d->BP1) {
+        std::fill(d->BP1, d->BP1 + c->nx, 0.0);
+        for (int j = 0; j < c->ny; j++)
+            for (int i = 0; i < c->nx; i++) d->BP1[i] += lp[(size_t)j * c->nx + i];
+    }
     if (d->BP2) for (int j = 0; j < c->ny; j++) { double s = 0; for (int i = 0; i < c->nx; i++) s += lp[(size_t)j * c->nx + i]; d->BP2[j] = s; }
     d->rho_peak = 0.0;
     if (c->p.plasm == 1) {
@@ -1450,7 +1456,9 @@ template <class T> static int adaptive(mpc_ctx *c, double *z, double *dzz, int *
         HIPCHK(hipStreamSynchronize(c->stream));
         if (c->timing) resolve_timing(c);
         /* BP and calc1oes of C2 for the step-size cap (propagation.c:343-362) */
-        for (int i = 0; i < c->nx; i++) { double sm = 0; for (int j = 0; j < c->ny; j++) sm += lp[(size_t)j * c->nx + i]; BP1[i] = sm; }
+        std::fill(BP1.begin(), BP1.end(), 0.0);          /* row by row; each BP1[i] adds in j order */
+        for (int j = 0; j < c->ny; j++)
+            for (int i = 0; i < c->nx; i++) BP1[i] += lp[(size_t)j * c->nx + i];
         for (int j = 0; j < c->ny; j++) { double sm = 0; for (int i = 0; i < c->nx; i++) sm += lp[(size_t)j * c->nx + i]; BP2[j] = sm; }
         int dims[3] = {c->nx, c->ny, c->nt};
         calc1oes(BP1.data(), dims, c->r.data(), &bps, y1);
